@@ -1,0 +1,184 @@
+// Internal declarations of libdvide_hip.so: the device-resident scene and the
+// host-side launchers of the collision, assembly, solver and update kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dvide.h"
+
+namespace dv {
+
+// Throwing HIP check used inside the library; the C ABI converts to codes.
+struct HipError {
+  hipError_t code;
+  std::string where;
+};
+struct DvError {
+  int code;
+  std::string msg;
+};
+#define DV_HIP(call)                                                         \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) throw ::dv::HipError{e_, std::string(#call)};      \
+  } while (0)
+
+// A growable device array.
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  void reserve(size_t n) {
+    if (n <= cap) return;
+    size_t want = n < 1024 ? 1024 : n + n / 2;
+    T* q = nullptr;
+    if (hipMalloc(&q, want * sizeof(T)) != hipSuccess)
+      throw DvError{DV_ERR_NOMEM, "device allocation of " + std::to_string(want * sizeof(T)) + " bytes failed"};
+    if (p) {
+      (void)hipFree(p);
+    }
+    p = q;
+    cap = want;
+  }
+  // grow keeping the first `keep` elements (stream-ordered copy)
+  void reserveKeep(size_t n, size_t keep, hipStream_t st) {
+    if (n <= cap) return;
+    size_t want = n + n / 2;
+    T* q = nullptr;
+    if (hipMalloc(&q, want * sizeof(T)) != hipSuccess)
+      throw DvError{DV_ERR_NOMEM, "device allocation failed"};
+    if (p && keep) DV_HIP(hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, st));
+    DV_HIP(hipStreamSynchronize(st));
+    if (p) (void)hipFree(p);
+    p = q;
+    cap = want;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Bodies with more incidences than this are reduced by a whole workgroup.
+constexpr int kHeavyDegree = 128;
+
+constexpr int kSlots = 8;  // reduction slots of the solver kernels
+
+struct ApgdState;
+struct SolveDims;
+
+// An instantiated hipGraph and the launch parameters it was captured with.
+struct GraphCache {
+  hipGraphExec_t exec = nullptr;
+  std::vector<char> key;
+  void reset();
+  bool matches(const void* k, size_t n) const;
+  void setKey(const void* k, size_t n);
+};
+
+struct Scene {
+  // --- parameters (System.cu:14-28, APGD.cu:6-16, CollisionDetector.cu:503-514)
+  double h = 1e-3, mu = 0.25, tol = 1e-4, envelope = 1e-3;
+  int maxIt = 100000;
+  double g[3] = {0, -9.81, 0};
+  int bins[3] = {20, 20, 20};
+  bool warm = false;
+  int solver = DV_SOLVER_APGD_FUSED;
+  int geomMode = 1;
+  int verbose = 0;
+  int grid = 0;
+  bool noGraph = false;  // launch the solver passes directly (profiling)
+  double time = 0;
+  int timeIndex = 0;
+  bool initialized = false;
+  std::string err;
+
+  // --- host staging (System::bodies and the constraint lists)
+  std::vector<double> hPos, hVel, hMass, hGeom;
+  std::vector<int> hFixed, hFamily;
+  std::vector<int> hBilA, hBilB;
+  std::vector<double> hBilVel, hBilT0, hBilP0;
+
+  // --- device state
+  hipStream_t stream = nullptr;
+  int device = 0, numCU = 0;
+  int nb = 0, nbil = 0;
+  int nsmall = 0, nbig = 0;
+  double cellSize = 0;
+  unsigned hashSize = 0;
+  DevBuf<double> p, v, f, k, fContact, tmp, minv;  // minv per body
+  DevBuf<double> geom;                             // 3 per body
+  DevBuf<int> family, smallIds, bigIds;
+  // bilateral rows
+  DevBuf<int> bilA, bilB;
+  DevBuf<double> bilVel, bilT0, bilP0;
+  DevBuf<int> bilIncStart, bilInc;  // per body: (row<<3)|(comp<<1)|sign
+
+  // --- collision detection work
+  DevBuf<unsigned> slotOf, slotCount, slotStart, slotFill;
+  DevBuf<int> cellItems;
+  DevBuf<double> cellPosR;  // 4 per sorted small geometry
+  DevBuf<int> cntUp, cntDown, cStart, iStart, scanTmp;
+  DevBuf<int> counters;     // [0] nc, [1] ninc, [2] nHeavy
+  DevBuf<int> hostTotals;
+  int* pinned = nullptr;    // pinned host copy of counters
+
+  // --- contacts (canonical (idA, idB) order)
+  int nc = 0, ncCap = 0;
+  DevBuf<int> cA, cB;
+  DevBuf<double> frame;  // 9 SoA arrays of ncCap: n, u, v components
+  DevBuf<double> gap;    // w = -penetration
+  DevBuf<double> loc;    // 3 SoA arrays of ncCap
+  DevBuf<float> frame32; // FP32 copy for the mixed solver
+  DevBuf<int> inc;       // per-body incidences (contact << 1 | side)
+  DevBuf<int> heavy;     // bodies processed by a whole workgroup
+
+  // --- system of the step (flat rows: nbil bilateral, then 3 per contact)
+  int R = 0, gammaLen = 0;
+  DevBuf<double> r, b, gamma;
+  // fused solver work
+  DevBuf<double> work;   // ring[3], ringN[3], J, gHat, tmpS  (FP64)
+  DevBuf<float> work32;  // same for the mixed solver
+  DevBuf<double> partials;
+  ApgdState* state = nullptr;      // device-resident APGD state
+  SolveDims* solveDims = nullptr;
+  ApgdState* stateHost = nullptr;  // pinned
+  SolveDims* dimsHost = nullptr;   // pinned
+  GraphCache graphFp64, graphMixed;
+
+  // --- statistics of the last step
+  dv_stats stats{};
+  hipEvent_t ev[6] = {};
+};
+
+// collision detection + incidence lists (dv_collide.hip)
+void collideInit(Scene& s);
+void collide(Scene& s);
+
+// assembly of k and r, update (dv_assemble.hip)
+void buildK(Scene& s);
+void buildR(Scene& s);
+void updateState(Scene& s, bool haveRows);
+void schurExact(Scene& s, const double* x, double* y);  // device pointers, flat R
+void gatherExact(Scene& s, const double* x, double* fOut, double* tmpOut, bool skipFixed);
+void ensureRowBuffers(Scene& s);
+void fillDevice(Scene& s, double* x, long n, double v);
+void rowsD(Scene& s, const double* x, const double* badd, double* y);
+
+// solvers (dv_apgd_plain.hip, dv_apgd_fused.hip)
+void solvePlain(Scene& s);
+void solveFused(Scene& s, bool mixed);
+
+// scans (dv_scan.hip): exclusive scan of n ints, out has n+1 entries
+void exclusiveScan(Scene& s, const int* in, int* out, int n);
+void exclusiveScanU(Scene& s, const unsigned* in, unsigned* out, int n);
+
+inline int blocksFor(long n, int t) { return (int)((n + t - 1) / t) > 0 ? (int)((n + t - 1) / t) : 1; }
+
+}  // namespace dv

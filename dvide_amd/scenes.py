@@ -1,0 +1,128 @@
+"""Seeded scene generators (ctypes binding of libdvide_scenes.so).
+
+The generators play the role of the reference's scenario drivers
+(models/test_compaction.cu:272-320, models/test_oneBall.cu:241-247): they
+produce body arrays that are then added to a System.  All randomness comes
+from std::mt19937_64 with u = (x >> 11) * 2^-53, so a (kind, seed, args)
+triple names one scene on any host.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+BALL_ON_BOX, LATTICE_BOX, COMPACTION, PILE, CLOUD, FREE_FALL = range(6)
+
+# slots of the parameter block written by the generator (scenes.cpp)
+P_H, P_MU, P_TOL, P_MAXIT, P_GX, P_GY, P_GZ, P_ENVELOPE, P_BINS, P_STEPS, P_WARM = range(11)
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libdvide_scenes.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+        lib = ctypes.CDLL(path)
+        lib.dvs_generate.restype = ctypes.c_void_p
+        lib.dvs_generate.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+        lib.dvs_free.argtypes = [ctypes.c_void_p]
+        for name in ("dvs_num_bodies", "dvs_num_bilaterals"):
+            getattr(lib, name).argtypes = [ctypes.c_void_p]
+            getattr(lib, name).restype = ctypes.c_int
+        for name in ("dvs_pos", "dvs_vel", "dvs_mass", "dvs_geom", "dvs_bil_info", "dvs_params"):
+            getattr(lib, name).argtypes = [ctypes.c_void_p]
+            getattr(lib, name).restype = ctypes.POINTER(ctypes.c_double)
+        for name in ("dvs_fixed", "dvs_family", "dvs_bil_dofs"):
+            getattr(lib, name).argtypes = [ctypes.c_void_p]
+            getattr(lib, name).restype = ctypes.POINTER(ctypes.c_int)
+        _LIB = lib
+    return _LIB
+
+
+@dataclass
+class Scene:
+    """Body arrays of one scene plus its solver parameters."""
+
+    pos: np.ndarray            # (n,3) float64
+    vel: np.ndarray            # (n,3) float64
+    mass: np.ndarray           # (n,)  float64
+    geom: np.ndarray           # (n,3) float64: (r,0,0) sphere, (hx,hy,hz) box
+    fixed: np.ndarray          # (n,)  int32
+    family: np.ndarray         # (n,)  int32
+    bil_dofs: np.ndarray       # (m,2) int32  (dofA, dofB or -1)
+    bil_info: np.ndarray       # (m,2) float64 (velocity, start time)
+    params: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.mass.shape[0])
+
+
+def _arr(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, (n,)).astype(dtype, copy=True)
+
+
+def generate(kind: int, seed: int, *args: float) -> Scene:
+    """Build a scene; see dvs_generate in scenes.cpp for the per-kind args."""
+    lib = _lib()
+    a = (ctypes.c_double * 8)(*(list(args) + [0.0] * (8 - len(args))))
+    h = lib.dvs_generate(int(kind), ctypes.c_uint64(seed), a)
+    if not h:
+        raise ValueError(f"unknown scene kind {kind}")
+    try:
+        n = lib.dvs_num_bodies(h)
+        m = lib.dvs_num_bilaterals(h)
+        prm = _arr(lib.dvs_params(h), 16, np.float64)
+        sc = Scene(
+            pos=_arr(lib.dvs_pos(h), 3 * n, np.float64).reshape(n, 3),
+            vel=_arr(lib.dvs_vel(h), 3 * n, np.float64).reshape(n, 3),
+            mass=_arr(lib.dvs_mass(h), n, np.float64),
+            geom=_arr(lib.dvs_geom(h), 3 * n, np.float64).reshape(n, 3),
+            fixed=_arr(lib.dvs_fixed(h), n, np.int32),
+            family=_arr(lib.dvs_family(h), n, np.int32),
+            bil_dofs=_arr(lib.dvs_bil_dofs(h), 2 * m, np.int32).reshape(m, 2),
+            bil_info=_arr(lib.dvs_bil_info(h), 2 * m, np.float64).reshape(m, 2),
+        )
+        sc.params = dict(
+            h=prm[P_H], mu=prm[P_MU], tol=prm[P_TOL], max_iterations=int(prm[P_MAXIT]),
+            gravity=(prm[P_GX], prm[P_GY], prm[P_GZ]), envelope=prm[P_ENVELOPE],
+            bins=int(prm[P_BINS]), steps=int(prm[P_STEPS]), warm_start=bool(prm[P_WARM]),
+        )
+        return sc
+    finally:
+        lib.dvs_free(h)
+
+
+def ball_on_box(gx: float = 0.0, h: float = 1e-2) -> Scene:
+    return generate(BALL_ON_BOX, 0, gx, h)
+
+
+def lattice_box(seed: int = 1001, per_side: int = 10) -> Scene:
+    """1,000 spheres r=0.05 rho=2500 mu=0.25 on a jittered lattice in a 1.2 m box."""
+    return generate(LATTICE_BOX, seed, per_side)
+
+
+def compaction(seed: int = 2002, n: int = 112000, width: float = 5.0) -> Scene:
+    """Settled granular pile under a lid driven down at 0.1 m/s (~250k contacts at 112k)."""
+    return generate(COMPACTION, seed, n, width)
+
+
+def pile(seed: int, target_contacts: int, rmin=0.03, rmax=0.06, mu=0.3) -> Scene:
+    return generate(PILE, seed, target_contacts, rmin, rmax, mu)
+
+
+def cloud(seed: int, n: int = 500, edge: float = 1.0, rmin=0.02, rmax=0.06) -> Scene:
+    return generate(CLOUD, seed, n, edge, rmin, rmax)
+
+
+def free_fall(seed: int, n: int = 10) -> Scene:
+    return generate(FREE_FALL, seed, n)

@@ -1,0 +1,173 @@
+/*
+ * dvide.h -- C ABI of libdvide_hip.so, the MI355X (gfx950) contact time step.
+ *
+ * The reference exposes this path as C++ classes only (System.cuh:279-326,
+ * Solver.cuh:17-46, APGD.cuh:62-67, Body.cuh:28-99, CollisionDetector.cuh:48-103);
+ * it has no C ABI (DanVIApi.h:7-20 defines export macros that are applied to
+ * nothing).  Each entry point below names the reference member it replaces.
+ * The C++ drop-in surface (dvide/System.h and friends) is a thin layer over
+ * these functions.
+ *
+ * Conventions
+ *   - every function returns int: 0 on success, a negative DV_ERR_* code on
+ *     failure; dv_last_error() gives the message for the last failure on a
+ *     scene;
+ *   - all buffers are caller-owned host memory, plain pointers and sizes;
+ *   - state vectors use the reference's flat layout: p, v, f_contact have
+ *     3 entries per body (DOF 3*j .. 3*j+2 of body j, System.cu:342-424);
+ *     gamma has the bilateral rows first, then 3 rows (n,u,v) per contact
+ *     (System.cu:1380, APGD.cu:166);
+ *   - one HIP stream per scene handle; calls on distinct handles may come from
+ *     distinct host threads, calls on one handle must not overlap.
+ */
+#ifndef DVIDE_H_
+#define DVIDE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dv_scene dv_scene;
+
+/* error codes */
+#define DV_OK 0
+#define DV_ERR_ARG (-1)         /* invalid argument */
+#define DV_ERR_STATE (-2)       /* call not valid in the scene's state */
+#define DV_ERR_HIP (-3)         /* a HIP runtime call failed */
+#define DV_ERR_NOMEM (-4)       /* device allocation failed or budget exceeded */
+#define DV_ERR_UNSUPPORTED (-5) /* option not supported by the selected solver */
+#define DV_ERR_DIVERGED (-6)    /* solver produced a non-finite value */
+
+/* solver implementations (the reference's factory, System.cu:77-101, knows
+ * 1=APGD .. 7=PJKIP; only APGD is on this path, in three builds) */
+#define DV_SOLVER_APGD_FUSED 1 /* persistent cooperative kernel, FP64      */
+#define DV_SOLVER_APGD_MIXED 2 /* persistent kernel, FP32 storage, FP64 tests */
+#define DV_SOLVER_APGD_PLAIN 3 /* host-driven FP64, one kernel per CUSP op  */
+
+/* double parameters (dv_set_param / dv_get_param) */
+#define DV_PARAM_H 0          /* time step              System::setTimeStep, System.cu:135 */
+#define DV_PARAM_MU 1         /* friction coefficient   System::setFrictionCoefficient, System.cu:140 */
+#define DV_PARAM_TOL 2        /* APGD tolerance         Solver::tolerance, Solver.cuh:23 */
+#define DV_PARAM_MAX_ITER 3   /* APGD iteration cap     Solver::maxIterations, Solver.cuh:24 */
+#define DV_PARAM_GX 4         /* gravity                System::gravity, System.cu:16 */
+#define DV_PARAM_GY 5
+#define DV_PARAM_GZ 6
+#define DV_PARAM_ENVELOPE 7   /* collision envelope     CollisionDetector::setEnvelope, CollisionDetector.cu:525 */
+#define DV_PARAM_WARM_START 8 /* 0/1                    APGD::setWarmStarting, APGD.cuh:62 */
+#define DV_PARAM_SOLVER 9     /* DV_SOLVER_*            System(int solverType), System.cu:62 */
+#define DV_PARAM_GEOM_MODE 10 /* 1: lengths/normals rounded through binary32 as the
+                                 reference's double3 helpers compile under nvcc
+                                 (helper_math_double.h:576-598); 0: plain FP64 */
+#define DV_PARAM_TIME 11      /* System::time (read/write) */
+#define DV_PARAM_VERBOSE 12   /* Solver::verbose */
+#define DV_PARAM_GRID 13      /* workgroups of the persistent solver (0 = auto) */
+
+/* state vectors (dv_get_vector / dv_set_vector) */
+#define DV_VEC_P 0         /* System::p_d        3*nb */
+#define DV_VEC_V 1         /* System::v_d        3*nb */
+#define DV_VEC_GAMMA 2     /* System::gamma_d    nbil + 3*nc */
+#define DV_VEC_F_CONTACT 3 /* System::f_contact_d 3*nb */
+#define DV_VEC_R 4         /* System::r_d        nbil + 3*nc */
+#define DV_VEC_K 5         /* System::k_d        3*nb */
+#define DV_VEC_B 6         /* System::b_d        nbil + 3*nc */
+
+/* per-step statistics (dv_get_stats) */
+typedef struct dv_stats {
+  int num_contacts;       /* CollisionDetector::numCollisions */
+  int num_bilaterals;     /* constraintsBilateralDOF_d.size() */
+  int iterations;         /* Solver::iterations (APGD.cu:357) */
+  int backtracks;         /* passes of the Lipschitz backtracking loop */
+  double residual;        /* best APGD residual (APGD.cu:293-301) */
+  double objective;       /* System::objectiveCCP (APGD.cu:353-354) */
+  double ms_total;        /* System::elapsedTime (System.cu:586-590) */
+  double ms_collision;    /* per-phase device time */
+  double ms_assembly;
+  double ms_solve;
+  double ms_update;
+  double time;            /* System::time after the step */
+} dv_stats;
+
+/* --- scene lifetime ------------------------------------------------------ */
+
+/* System::System(int solverType), System.cu:62-133 */
+int dv_scene_create(dv_scene** out, int solver);
+int dv_scene_destroy(dv_scene* s);
+const char* dv_last_error(const dv_scene* s);
+
+/* System::add(Body*), System.cu:145-150, with the Body fields of Body.cuh:45-98:
+ * pos[3*n], vel[3*n], mass[n], geom[3*n] ((r,0,0) sphere, (hx,hy,hz) box),
+ * fixed[n] (Body::setBodyFixed), family[n] (PhysicsItem::setCollisionFamily).
+ * Bodies get consecutive identifiers starting at the current body count. */
+int dv_add_bodies(dv_scene* s, int n, const double* pos, const double* vel, const double* mass,
+                  const double* geom, const int* fixed, const int* family);
+
+/* System::addBilateralConstraintDOF(int,int,double,double), System.cu:510-532:
+ * row p[dofA] - p[dofB] (or p[dofA] - p0 when dofB < 0) driven at `velocity`
+ * from `start_time` on. */
+int dv_add_bilateral_dof(dv_scene* s, int dofA, int dofB, double velocity, double start_time);
+
+/* CollisionDetector::setBinsPerAxis, CollisionDetector.cu:6 (accepted for API
+ * parity; the spatial-hash broadphase sizes its cells from the geometry). */
+int dv_set_bins_per_axis(dv_scene* s, int bx, int by, int bz);
+
+int dv_set_param(dv_scene* s, int param, double value);
+int dv_get_param(const dv_scene* s, int param, double* value);
+
+/* System::initializeSystem, System.cu:342-508: packs and uploads the bodies. */
+int dv_initialize(dv_scene* s);
+
+/* System::DoTimeStep, System.cu:544-602: collision detection, D and r
+ * assembly, APGD, and the update v = M^-1 (k + D^T gamma), p += h v. */
+int dv_step(dv_scene* s);
+
+/* --- data access (System's public device vectors) ------------------------- */
+
+int dv_num_bodies(const dv_scene* s, int* n);
+int dv_num_contacts(const dv_scene* s, int* nc);
+int dv_get_stats(const dv_scene* s, dv_stats* out);
+
+/* copies min(n, length) entries of the vector into out; *len gets its length */
+int dv_get_vector(dv_scene* s, int which, double* out, size_t n, size_t* len);
+/* overwrites p or v (drivers mutate p_d/v_d between steps, models/test_draft.cu:438-439) */
+int dv_set_vector(dv_scene* s, int which, const double* in, size_t n);
+
+/* CollisionDetector::collisionIdentifierA_d/B_d and normalsAndPenetrations_d
+ * (CollisionDetector.cu:495-498): idA < idB geometry indices, normal from B to A
+ * and the gap (= -penetration); loc (may be NULL) = collisionLocations_d.
+ * Arrays sized for nc entries (nap: 4*nc, loc: 3*nc), canonical (idA, idB) order. */
+int dv_get_contacts(dv_scene* s, uint32_t* idA, uint32_t* idB, double* nap, double* loc,
+                    size_t nc);
+
+/* System::getCCPViolation, System.cu:1605-1643: min(gamma_n,0), min((Dv+b)_n,0),
+ * gamma_n . (Dv+b)_n and the 2-norm of min(mu gamma_n - |gamma_t|, 0). */
+int dv_ccp_violation(dv_scene* s, double out[4]);
+
+/* --- hot-path pieces, exposed for parity tests ---------------------------- */
+
+/* Collision detection only (CollisionDetector::generateAxisAlignedBoundingBoxes,
+ * detectPossibleCollisions_spatialSubdivision, detectCollisions,
+ * CollisionDetector.cu:547-665) plus assembly of k, D and r
+ * (System::buildAppliedImpulseVector / buildContactJacobian / buildSchurVector,
+ * System.cu:1351-1409, 1502-1580).  *nc gets the contact count. */
+int dv_assemble(dv_scene* s, int* nc);
+
+/* y = N x = D M^-1 D^T x with the assembled D (APGD::performSchurComplementProduct,
+ * APGD.cu:81-87); x, y have nbil + 3*nc entries.  Terms are summed in D^T's
+ * row order, as a row-sorted COO SpMV sums them. */
+int dv_schur_product(dv_scene* s, const double* x, double* y, size_t n);
+
+/* Coulomb-cone projection of n 3-vectors in place with friction mu
+ * (APGD.cu:47-78). */
+int dv_project_cone(dv_scene* s, double* g, size_t n, double mu);
+
+/* APGD::solve, APGD.cu:164-361, on the assembled system (no update). */
+int dv_solve(dv_scene* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DVIDE_H_ */

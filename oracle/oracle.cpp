@@ -1,0 +1,865 @@
+// ORACLE -- test infrastructure only.
+//
+// A CPU, FP64 restatement of DVIDE's contact time step, used as the checker
+// for the HIP implementation in dvide_amd/.  Only tests/, __graft_entry__.smoke()
+// and bench.py's cpu_baseline leg may load it; the product never does.
+//
+// It follows the reference's algorithm operation by operation (citations are
+// to files of melanz/DVIDE):
+//   System::initializeSystem / processConstraints   System.cu:303-340, 342-424
+//   CollisionDetector (AABB, uniform-grid broadphase,
+//     sphere-sphere / sphere-box narrowphase)      CollisionDetector.cu:10-185, 187-501, 547-665
+//   buildAppliedImpulseVector / multiplyByMass     System.cu:1411-1417, 1502-1514
+//   buildContactJacobian (+ bilateral rows)        System.cu:622-638, 676-767, 904-967, 1321-1409
+//   buildSchurVector / buildStabilization          System.cu:1516-1544, 1561-1580
+//   APGD::solve, project, getResidual              APGD.cu:47-122, 164-361
+//   DoTimeStep update                              System.cu:544-602
+//   getCCPViolation                                System.cu:1590-1643
+//
+// Linear algebra mirrors CUSP semantics: D and D^T are COO matrices sorted by
+// row (D^T by a stable sort, as cusp::coo_matrix::sort_by_row), and a COO
+// SpMV y = A x accumulates each row's entries in storage order.  Dot products
+// and norms are summed in the fixed CR256 order (see cr256), so results do not
+// depend on the OpenMP thread count.  Built with -ffp-contract=off so that no
+// multiply-add is fused: the narrowphase keep test and contact frames are then
+// evaluated exactly as the HIP kernels evaluate them.
+//
+// Pinned by (tests/test_oracle.py): brute-force all-pairs contact sets and
+// closed-form sphere/box geometry, the closed form of the Coulomb-cone
+// projection and a numerical nearest-point search, dense D M^-1 D^T products,
+// free fall by semi-implicit Euler, the ball-on-box normal impulse m g h and
+// the stick/slip threshold |g_x| = mu g, the pile weight balance, and the CCP
+// optimality conditions at convergence.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+struct V3 {
+  double x, y, z;
+};
+inline V3 mk(double x, double y, double z) { return V3{x, y, z}; }
+inline V3 operator+(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 operator-(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 operator-(V3 a) { return mk(-a.x, -a.y, -a.z); }
+inline V3 operator*(V3 a, double s) { return mk(a.x * s, a.y * s, a.z * s); }
+inline V3 operator*(double s, V3 a) { return mk(s * a.x, s * a.y, s * a.z); }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// length / normalize.  The reference's double3 helpers call sqrtf/rsqrtf
+// (helper_math_double.h:576-579, 594-598), which nvcc resolves to the binary32
+// functions: lengths and unit normals carry binary32 rounding.  Mode 1
+// restates that (correctly rounded binary32 sqrt and reciprocal, computed via
+// double operations whose second rounding is innocuous); mode 0 is plain FP64.
+int g_geomMode = 1;
+inline double lengthOf(V3 v) {
+  double d = dot(v, v);
+  if (g_geomMode == 1) return (double)(float)std::sqrt((double)(float)d);
+  return std::sqrt(d);
+}
+inline double invLengthOf(V3 v) {
+  double d = dot(v, v);
+  if (g_geomMode == 1) {
+    float s = (float)std::sqrt((double)(float)d);
+    return (double)(float)(1.0 / (double)s);
+  }
+  return 1.0 / std::sqrt(d);
+}
+inline V3 normalize(V3 v) { return v * invLengthOf(v); }
+
+// Deterministic reductions in a fixed order ("CR256"): the vector is cut
+// into consecutive blocks of 256 terms (zero-padded); each block is reduced
+// by a binary tree with strides 128, 64, ..., 1 (t[i] += t[i + s] for i < s);
+// the block results are then added left to right.  A 256-thread GPU block
+// reproduces this order exactly, which makes the plain FP64 HIP path's dot
+// products bit-identical to these.
+template <class F>
+double cr256(size_t n, F term) {
+  const size_t nblk = (n + 255) / 256;
+  std::vector<double> part(nblk, 0.0);
+#pragma omp parallel for schedule(static)
+  for (long blk = 0; blk < (long)nblk; ++blk) {
+    double t[256];
+    for (int i = 0; i < 256; ++i) {
+      size_t q = (size_t)blk * 256 + i;
+      t[i] = q < n ? term(q) : 0.0;
+    }
+    for (int st = 128; st >= 1; st >>= 1)
+      for (int i = 0; i < st; ++i) t[i] = t[i] + t[i + st];
+    part[blk] = t[0];
+  }
+  double s = 0;
+  for (double p : part) s += p;
+  return s;
+}
+
+double vdot(const std::vector<double>& a, const std::vector<double>& b) {
+  return cr256(a.size(), [&](size_t i) { return a[i] * b[i]; });
+}
+double vnrm2(const std::vector<double>& a) {
+  return std::sqrt(cr256(a.size(), [&](size_t i) { return a[i] * a[i]; }));
+}
+double vnrmmax(const std::vector<double>& a) {
+  double m = 0;
+  for (double x : a) m = std::max(m, std::fabs(x));
+  return m;
+}
+// z = a*x + b*y   (cusp::blas::axpby)
+void axpby(const std::vector<double>& x, const std::vector<double>& y, std::vector<double>& z,
+           double a, double b) {
+  z.resize(x.size());
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)x.size(); ++i) z[i] = a * x[i] + b * y[i];
+}
+
+// COO matrix sorted by row, with a row-pointer index for the SpMV.
+struct Coo {
+  int rows = 0, cols = 0;
+  std::vector<int> I, J;
+  std::vector<double> V;
+  std::vector<size_t> rowPtr;
+  void index() {
+    rowPtr.assign(rows + 1, 0);
+    for (int r : I) rowPtr[r + 1]++;
+    for (int r = 0; r < rows; ++r) rowPtr[r + 1] += rowPtr[r];
+  }
+  // y = A x
+  void mul(const std::vector<double>& x, std::vector<double>& y) const {
+    y.assign(rows, 0.0);
+#pragma omp parallel for schedule(static)
+    for (long r = 0; r < rows; ++r) {
+      double s = 0;
+      for (size_t k = rowPtr[r]; k < rowPtr[r + 1]; ++k) s += V[k] * x[J[k]];
+      y[r] = s;
+    }
+  }
+};
+
+struct Aabb {
+  V3 lo, hi;
+};
+
+struct System {
+  // parameters (System.cu:14-28, CollisionDetector.cu:503-514, APGD.cu:6-16)
+  double h = 1e-3, mu = 0.25, tol = 1e-4, envelope = 1e-3;
+  int maxIt = 100000;
+  V3 gravity = mk(0, -9.81, 0);
+  unsigned bins[3] = {20, 20, 20};
+  bool warm = false, antiRelax = false;
+  double time = 0;
+  int timeIndex = 0;
+
+  // bodies
+  int nb = 0;
+  std::vector<double> p, v, f, fEl, k, fContact, tmp, minv;  // minv per DOF
+  std::vector<double> geom;                                  // 3 per body
+  std::vector<int> fixed, family;
+  bool initialized = false;
+
+  // bilateral DOF constraints (System.cu:510-532)
+  std::vector<int> bilA, bilB, bilOffset;
+  std::vector<double> bilVel, bilT0, bilP0;
+  int bilNnz = 0;
+
+  // collision detection output
+  std::vector<uint32_t> idA, idB;
+  std::vector<double> nAndP;  // 4 per contact: -normal (B to A), -penetration
+  std::vector<double> loc;    // 3 per contact
+  size_t numPossible = 0;
+  int nc = 0;
+
+  // system of the step
+  Coo D, DT, M;
+  std::vector<double> friction, r, b, gamma;
+
+  // solver outputs
+  int iterations = 0;
+  double residual = 0, objective = 0;
+  int backtracks = 0;
+
+  int rows() const { return 3 * nc + (int)bilA.size(); }
+
+  // --- System::initializeSystem (bodies) and processConstraints -------------
+  void initialize() {
+    nb = (int)fixed.size();
+    const int n3 = 3 * nb;
+    f.assign(n3, 0.0);
+    fEl.assign(n3, 0.0);
+    k.assign(n3, 0.0);
+    fContact.assign(n3, 0.0);
+    tmp.assign(n3, 0.0);
+    M.rows = M.cols = n3;
+    M.I.resize(n3); M.J.resize(n3); M.V.resize(n3);
+    minv.resize(n3);
+    for (int j = 0; j < nb; ++j) {
+      for (int c = 0; c < 3; ++c) {
+        double g = c == 0 ? gravity.x : (c == 1 ? gravity.y : gravity.z);
+        f[3 * j + c] = fixed[j] ? 0.0 : bodyMass[j] * g;
+        M.I[3 * j + c] = M.J[3 * j + c] = 3 * j + c;
+        M.V[3 * j + c] = fixed[j] ? 0.0 : 1.0 / bodyMass[j];
+        minv[3 * j + c] = M.V[3 * j + c];
+      }
+    }
+    M.index();
+    gamma.assign(n3, 0.0);  // gamma_d = a_h (System.cu:185): zeros, one per DOF
+    bilOffset.clear();
+    bilP0.assign(bilA.size(), 0.0);
+    int off = 0;
+    for (size_t i = 0; i < bilA.size(); ++i) {
+      bilOffset.push_back(off);
+      if (bilB[i] < 0) {
+        off += 1;
+        bilP0[i] = p[bilA[i]];
+      } else {
+        off += 2;
+      }
+    }
+    bilNnz = off;
+    initialized = true;
+  }
+  std::vector<double> bodyMass;
+
+  V3 bodyPos(int g) const { return mk(p[3 * g], p[3 * g + 1], p[3 * g + 2]); }
+  V3 halfExt(int g) const {
+    V3 e = mk(geom[3 * g], geom[3 * g + 1], geom[3 * g + 2]);
+    if (e.y == 0) e = mk(e.x, e.x, e.x);
+    return e;
+  }
+  bool isSphere(int g) const { return geom[3 * g + 1] == 0; }
+
+  // --- generateAabbData (CollisionDetector.cu:24-79) ------------------------
+  std::vector<Aabb> aabbs() const {
+    std::vector<Aabb> a(nb);
+    for (int g = 0; g < nb; ++g) {
+      V3 pos = bodyPos(g), e = halfExt(g);
+      e = e + mk(envelope, envelope, envelope);
+      a[g].lo = pos - e;
+      a[g].hi = pos + e;
+    }
+    return a;
+  }
+
+  // --- detectPossibleCollisions_spatialSubdivision (CollisionDetector.cu:555-631)
+  std::vector<uint64_t> broadphase() {
+    std::vector<Aabb> a = aabbs();
+    std::vector<uint64_t> keys;
+    if (nb == 0) return keys;
+    V3 lo = a[0].lo, hi = a[0].lo;
+    for (const Aabb& b : a) {
+      for (V3 q : {b.lo, b.hi}) {
+        lo = mk(std::min(lo.x, q.x), std::min(lo.y, q.y), std::min(lo.z, q.z));
+        hi = mk(std::max(hi.x, q.x), std::max(hi.y, q.y), std::max(hi.z, q.z));
+      }
+    }
+    lo = lo - mk(0.01, 0.01, 0.01);
+    hi = hi + mk(0.01, 0.01, 0.01);
+    V3 bsi = mk((double)bins[0] / std::fabs(hi.x - lo.x), (double)bins[1] / std::fabs(hi.y - lo.y),
+                (double)bins[2] / std::fabs(hi.z - lo.z));
+    for (Aabb& b : a) {
+      b.lo = b.lo - lo;
+      b.hi = b.hi - lo;
+    }
+    // (bin, aabb) pairs, then group by bin
+    std::vector<std::pair<uint32_t, uint32_t>> bi;
+    for (int g = 0; g < nb; ++g) {
+      unsigned x0 = (unsigned)(a[g].lo.x * bsi.x), y0 = (unsigned)(a[g].lo.y * bsi.y),
+               z0 = (unsigned)(a[g].lo.z * bsi.z);
+      unsigned x1 = (unsigned)(a[g].hi.x * bsi.x), y1 = (unsigned)(a[g].hi.y * bsi.y),
+               z1 = (unsigned)(a[g].hi.z * bsi.z);
+      for (unsigned i = x0; i <= x1; ++i)
+        for (unsigned j = y0; j <= y1; ++j)
+          for (unsigned kk = z0; kk <= z1; ++kk)
+            bi.push_back({i + j * bins[0] + kk * bins[0] * bins[1], (uint32_t)g});
+    }
+    std::stable_sort(bi.begin(), bi.end(),
+                     [](const std::pair<uint32_t, uint32_t>& x,
+                        const std::pair<uint32_t, uint32_t>& y) { return x.first < y.first; });
+    size_t s = 0;
+    while (s < bi.size()) {
+      size_t e = s;
+      while (e < bi.size() && bi[e].first == bi[s].first) ++e;
+      for (size_t i = s; i < e; ++i) {
+        uint32_t ga = bi[i].second;
+        const Aabb& A = a[ga];
+        for (size_t kk = i + 1; kk < e; ++kk) {
+          uint32_t gb = bi[kk].second;
+          const Aabb& B = a[gb];
+          bool in = (A.lo.x <= B.hi.x && B.lo.x <= A.hi.x) &&
+                    (A.lo.y <= B.hi.y && B.lo.y <= A.hi.y) &&
+                    (A.lo.z <= B.hi.z && B.lo.z <= A.hi.z);
+          if (ga == gb) in = false;  // same body (collisionMap.x equal)
+          if (family[ga] == family[gb] && family[ga] != -1) in = false;
+          if (in) {
+            uint64_t lo32 = std::min(ga, gb), hi32 = std::max(ga, gb);
+            keys.push_back((lo32 << 32) | hi32);
+          }
+        }
+      }
+      s = e;
+    }
+    std::stable_sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    return keys;
+  }
+
+  // --- narrowphase keep test and contact record
+  //     (countActualCollisions CollisionDetector.cu:284-328,
+  //      storeActualCollisions CollisionDetector.cu:432-498)
+  bool narrow(int ga, int gb, double* nOut, double* pen, V3* locOut) const {
+    V3 posA = bodyPos(ga), posB = bodyPos(gb);
+    V3 gA = mk(geom[3 * ga], geom[3 * ga + 1], geom[3 * ga + 2]);
+    V3 gB = mk(geom[3 * gb], geom[3 * gb + 1], geom[3 * gb + 2]);
+    V3 normal = mk(1, 0, 0), location = mk(0, 0, 0);
+    double penetration = 0;
+    if (gA.y == 0 && gB.y == 0) {
+      penetration = (gA.x + gB.x) - lengthOf(posB - posA);
+      if (!(penetration >= -envelope)) return false;
+      normal = normalize(posB - posA);  // from A to B
+      location = posA + normal * gA.x;
+    } else if ((gA.y != 0 && gB.y == 0) || (gA.y == 0 && gB.y != 0)) {
+      double dmin = 0, rr = gB.x;
+      V3 c = posB, bmin = posA - gA, bmax = posA + gA;
+      if (gA.y == 0 && gB.y != 0) {
+        rr = gA.x;
+        c = posA;
+        bmin = posB - gB;
+        bmax = posB + gB;
+      }
+      V3 nn = mk(0, 0, 0);
+      if (c.x < bmin.x) { dmin += (c.x - bmin.x) * (c.x - bmin.x); nn.x += c.x - bmin.x; }
+      else if (c.x > bmax.x) { dmin += (c.x - bmax.x) * (c.x - bmax.x); nn.x += c.x - bmax.x; }
+      if (c.y < bmin.y) { dmin += (c.y - bmin.y) * (c.y - bmin.y); nn.y += c.y - bmin.y; }
+      else if (c.y > bmax.y) { dmin += (c.y - bmax.y) * (c.y - bmax.y); nn.y += c.y - bmax.y; }
+      if (c.z < bmin.z) { dmin += (c.z - bmin.z) * (c.z - bmin.z); nn.z += c.z - bmin.z; }
+      else if (c.z > bmax.z) { dmin += (c.z - bmax.z) * (c.z - bmax.z); nn.z += c.z - bmax.z; }
+      penetration = rr - std::sqrt(dmin);
+      if (!(penetration >= -envelope)) return false;
+      normal = normalize(nn);
+      location = c - rr * normal;
+      if (gA.y == 0 && gB.y != 0) normal = -normal;
+    } else {
+      return false;  // box-box: no narrowphase
+    }
+    if (nOut) {
+      nOut[0] = -normal.x; nOut[1] = -normal.y; nOut[2] = -normal.z;  // from B to A
+      *pen = -penetration;                                             // the gap
+      *locOut = location;
+    }
+    return true;
+  }
+
+  void collide() {
+    std::vector<uint64_t> keys = broadphase();
+    numPossible = keys.size();
+    idA.clear(); idB.clear(); nAndP.clear(); loc.clear();
+    std::vector<char> keep(keys.size());
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)keys.size(); ++i)
+      keep[i] = narrow((int)(keys[i] >> 32), (int)(keys[i] & 0xffffffffu), nullptr, nullptr, nullptr);
+    for (size_t i = 0; i < keys.size(); ++i) {
+      if (!keep[i]) continue;
+      int ga = (int)(keys[i] >> 32), gb = (int)(keys[i] & 0xffffffffu);
+      double n[3], w;
+      V3 l;
+      narrow(ga, gb, n, &w, &l);
+      idA.push_back(ga);
+      idB.push_back(gb);
+      nAndP.insert(nAndP.end(), {n[0], n[1], n[2], w});
+      loc.insert(loc.end(), {l.x, l.y, l.z});
+    }
+    nc = (int)idA.size();
+  }
+
+  // --- buildAppliedImpulseVector (System.cu:1502-1514, 1411-1417) ------------
+  void buildK() {
+    const int n3 = 3 * nb;
+    for (int i = 0; i < n3; ++i) {
+      double m = minv[i];
+      if (m != 0) m = 1.0 / m;
+      k[i] = m * v[i];
+    }
+    for (int i = 0; i < n3; ++i) k[i] = (h * f[i] + (-h) * fEl[i]) + 1.0 * k[i];
+  }
+
+  // --- contact frame (System.cu:753-762) -------------------------------------
+  static void frame(const double* n3, V3* u, V3* w) {
+    V3 n = mk(n3[0], n3[1], n3[2]);
+    V3 uu;
+    if (n.z != 0) uu = normalize(mk(1, 0, -n.x / n.z));
+    else if (n.x != 0) uu = normalize(mk(-n.z / n.x, 0, 1));
+    else uu = normalize(mk(1, -n.x / n.y, 0));
+    *u = uu;
+    *w = normalize(cross(n, uu));
+  }
+
+  // --- buildContactJacobian + transpose (System.cu:622-638, 676-1257, 1351-1409)
+  void buildD() {
+    const int nbil = (int)bilA.size();
+    D = Coo();
+    D.rows = 3 * nc + nbil;
+    D.cols = 3 * nb;
+    const size_t nnz = (size_t)bilNnz + 18 * (size_t)nc;
+    D.I.resize(nnz); D.J.resize(nnz); D.V.resize(nnz);
+    for (int i = 0; i < nbil; ++i) {
+      int o = bilOffset[i];
+      D.I[o] = i; D.J[o] = bilA[i]; D.V[o] = 1.0;
+      if (bilB[i] >= 0) { D.I[o + 1] = i; D.J[o + 1] = bilB[i]; D.V[o + 1] = -1.0; }
+    }
+    friction.assign(nc, mu);
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < nc; ++c) {
+      size_t o = (size_t)bilNnz + 18 * (size_t)c;
+      V3 u, w;
+      frame(&nAndP[4 * c], &u, &w);
+      V3 n = mk(nAndP[4 * c], nAndP[4 * c + 1], nAndP[4 * c + 2]);
+      const V3 rowsv[3] = {n, u, w};
+      int jA = 3 * (int)idA[c], jB = 3 * (int)idB[c];
+      for (int rr = 0; rr < 3; ++rr) {
+        int row = 3 * (int)c + rr + nbil;
+        const double vals[3] = {rowsv[rr].x, rowsv[rr].y, rowsv[rr].z};
+        for (int q = 0; q < 3; ++q) {
+          D.I[o] = row; D.J[o] = jA + q; D.V[o] = vals[q]; ++o;
+        }
+        for (int q = 0; q < 3; ++q) {
+          D.I[o] = row; D.J[o] = jB + q; D.V[o] = -vals[q]; ++o;
+        }
+      }
+    }
+    D.index();
+    // D^T: swap indices and stable-sort by row
+    DT = Coo();
+    DT.rows = D.cols;
+    DT.cols = D.rows;
+    std::vector<size_t> perm(nnz);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t b) { return D.J[a] < D.J[b]; });
+    DT.I.resize(nnz); DT.J.resize(nnz); DT.V.resize(nnz);
+    for (size_t q = 0; q < nnz; ++q) {
+      DT.I[q] = D.J[perm[q]];
+      DT.J[q] = D.I[perm[q]];
+      DT.V[q] = D.V[perm[q]];
+    }
+    DT.index();
+  }
+
+  // --- buildSchurVector (System.cu:1516-1544, 1561-1580) ---------------------
+  void buildR() {
+    const int nbil = (int)bilA.size();
+    M.mul(k, tmp);
+    D.mul(tmp, r);
+    b.assign(rows(), 0.0);
+    for (int i = 0; i < nbil; ++i) {
+      double vel = bilVel[i], t0 = bilT0[i];
+      if (time < t0) vel = 0;
+      double viol = bilB[i] < 0 ? p[bilA[i]] - bilP0[i] - vel * (time - t0)
+                                : p[bilA[i]] - p[bilB[i]] - vel * (time - t0);
+      b[i] = viol / h;
+    }
+    for (int c = 0; c < nc; ++c) {
+      b[3 * c + nbil] = nAndP[4 * c + 3] / h;
+      b[3 * c + 1 + nbil] = 0;
+      b[3 * c + 2 + nbil] = 0;
+    }
+    for (size_t i = 0; i < r.size(); ++i) r[i] = 1.0 * b[i] + r[i];
+  }
+
+  // N x = D (M^-1 (D^T x))  (APGD.cu:81-87)
+  void schur(const std::vector<double>& x, std::vector<double>& y) {
+    DT.mul(x, fContact);
+    M.mul(fContact, tmp);
+    D.mul(tmp, y);
+  }
+
+  // project (APGD.cu:47-78) on a flat vector, contacts at offset `off`
+  static void projectOne(double* g, double m) {
+    double gn = g[0];
+    double gt = std::sqrt(g[1] * g[1] + g[2] * g[2]);
+    if (m == 0) {
+      g[1] = 0; g[2] = 0;
+      if (gn < 0) g[0] = 0;
+    } else if (gt < m * gn) {
+      // inside the cone
+    } else if ((gt < -(1.0 / m) * gn) || (std::fabs(gn) < 10e-15)) {
+      g[0] = 0; g[1] = 0; g[2] = 0;
+    } else {
+      double gnp = (gt * m + gn) / (m * m + 1.0);
+      double gtp = gnp * m;
+      double q = gtp / gt;
+      g[0] = gnp; g[1] = q * g[1]; g[2] = q * g[2];
+    }
+  }
+  void project(std::vector<double>& x) const {
+    const int off = (int)bilA.size();
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < nc; ++c) projectOne(&x[3 * c + off], friction[c]);
+  }
+
+  // getResidual (APGD.cu:90-102)
+  double getResidual(const std::vector<double>& src, std::vector<double>& t) {
+    double cnt = (double)((size_t)nc + bilA.size());
+    double gdiff = 1.0 / (cnt * cnt);
+    schur(src, t);
+    for (size_t i = 0; i < t.size(); ++i) t[i] = 1.0 * r[i] + t[i];
+    if (nc) {
+      axpby(src, t, t, 1.0, -gdiff);
+      project(t);
+      axpby(src, t, t, 1.0 / gdiff, -1.0 / gdiff);
+    }
+    return vnrmmax(t);
+  }
+
+  // APGD::solve (APGD.cu:164-361)
+  void apgd() {
+    const size_t R = (size_t)rows();
+    gamma.resize(R, 0.0);  // thrust resize: keeps the leading values, zero-fills
+    std::vector<double> gammaHat(R, 1.0), gammaNew(R), g(R), y(R), yNew(R), t(R);
+    if (!warm)
+      for (int c = 0; c < nc; ++c) { gamma[3 * c] = 1.0; gamma[3 * c + 1] = 0; gamma[3 * c + 2] = 0; }
+    y = gamma;
+    double theta = 1.0, thetaNew = 1.0, beta = 0, obj1 = 0, obj2 = 0, best = 10e30;
+    axpby(gamma, gammaHat, t, 1.0, -1.0);
+    double L = vnrm2(t);
+    std::vector<double> nt;
+    schur(t, nt);
+    L = vnrm2(nt) / L;
+    double tt = 1.0 / L;
+    backtracks = 0;
+    int it;
+    std::vector<double> Ny, Ng;
+    for (it = 0; it < maxIt; ++it) {
+      schur(y, Ny);
+      axpby(Ny, r, g, 1.0, 1.0);
+      axpby(y, g, gammaNew, 1.0, -tt);
+      project(gammaNew);
+      schur(gammaNew, Ng);
+      obj1 = 0.5 * vdot(gammaNew, Ng) + vdot(gammaNew, r);
+      obj2 = 0.5 * vdot(y, Ny) + vdot(y, r);
+      axpby(gammaNew, y, t, 1.0, -1.0);
+      double nr = vnrm2(t);
+      obj2 += vdot(g, t) + 0.5 * L * (nr * nr);
+      while (obj1 >= obj2) {
+        ++backtracks;
+        L = 2.0 * L;
+        tt = 1.0 / L;
+        axpby(y, g, gammaNew, 1.0, -tt);
+        project(gammaNew);
+        schur(gammaNew, Ng);
+        obj1 = 0.5 * vdot(gammaNew, Ng) + vdot(gammaNew, r);
+        obj2 = 0.5 * vdot(y, Ny) + vdot(y, r);
+        axpby(gammaNew, y, t, 1.0, -1.0);
+        nr = vnrm2(t);
+        obj2 += vdot(g, t) + 0.5 * L * (nr * nr);
+      }
+      thetaNew = (-(theta * theta) + theta * std::sqrt(theta * theta + 4.0)) / 2.0;
+      beta = theta * (1.0 - theta) / (theta * theta + thetaNew);
+      axpby(gammaNew, gamma, yNew, 1.0 + beta, -beta);
+      double res = getResidual(gammaNew, t);
+      if (res < best) {
+        best = res;
+        gammaHat = gammaNew;
+      }
+      if (best < tol) break;
+      axpby(gammaNew, gamma, t, 1.0, -1.0);
+      if (vdot(g, t) > 0) {
+        yNew = gammaNew;
+        thetaNew = 1.0;
+      }
+      L = 0.9 * L;
+      tt = 1.0 / L;
+      theta = thetaNew;
+      gamma = gammaNew;
+      y = yNew;
+      if (antiRelax) {
+        // APGD.cu:335-345
+        for (size_t i = 0; i < R; ++i) r[i] = -1.0 * antiRelaxVec[i] + r[i];
+        DT.mul(gamma, fContact);
+        for (int i = 0; i < 3 * nb; ++i) tmp[i] = k[i] + fContact[i];
+        M.mul(tmp, v);
+        D.mul(v, t);
+        const int off = (int)bilA.size();
+        (void)off;
+        // updateAntiRelaxationVector indexes s and antiRelaxation without the
+        // bilateral offset (APGD.cu:104-114)
+        for (int c = 0; c < nc; ++c) {
+          double sv = t[3 * c + 1], sw = t[3 * c + 2];
+          antiRelaxVec[3 * c] = std::sqrt(sv * sv + sw * sw) * friction[c];
+          antiRelaxVec[3 * c + 1] = 0;
+          antiRelaxVec[3 * c + 2] = 0;
+        }
+        for (size_t i = 0; i < R; ++i) r[i] = 1.0 * antiRelaxVec[i] + r[i];
+      }
+    }
+    schur(gammaHat, t);
+    objective = 0.5 * vdot(gammaHat, t) + vdot(gammaHat, r);
+    iterations = it;
+    residual = best;
+    gamma = gammaHat;
+  }
+  std::vector<double> antiRelaxVec;
+
+  // --- System::DoTimeStep (System.cu:544-602) --------------------------------
+  void step() {
+    collide();
+    buildK();
+    const int nbil = (int)bilA.size();
+    if (nc || nbil) {
+      buildD();
+      buildR();
+      antiRelaxVec.assign(rows(), 0.0);
+      apgd();
+      DT.mul(gamma, fContact);
+      for (int i = 0; i < 3 * nb; ++i) tmp[i] = 1.0 * k[i] + 1.0 * fContact[i];
+      M.mul(tmp, v);
+      for (int i = 0; i < 3 * nb; ++i) fContact[i] = (1.0 / h) * fContact[i];
+    } else {
+      M.mul(k, v);
+      std::fill(fContact.begin(), fContact.end(), 0.0);
+      iterations = 0;
+      residual = 0;
+    }
+    for (int i = 0; i < 3 * nb; ++i) p[i] = h * v[i] + p[i];
+    time += h;
+    timeIndex++;
+  }
+
+  // --- getCCPViolation (System.cu:1590-1643), contact rows at their offset ---
+  void ccpViolation(double out[4]) {
+    out[0] = out[1] = out[2] = out[3] = 0;
+    if (!nc) return;
+    const int off = (int)bilA.size();
+    std::vector<double> dv;
+    D.mul(v, dv);
+    double mn = 1e300, mv = 1e300, comp = 0, cone = 0;
+    for (int c = 0; c < nc; ++c) {
+      double gn = gamma[3 * c + off];
+      double vn = dv[3 * c + off] + b[3 * c + off];
+      mn = std::min(mn, gn);
+      mv = std::min(mv, vn);
+      comp += gn * vn;
+      double gt = std::sqrt(gamma[3 * c + 1 + off] * gamma[3 * c + 1 + off] +
+                            gamma[3 * c + 2 + off] * gamma[3 * c + 2 + off]);
+      double cv = friction[c] * gn - gt;
+      if (cv > 0) cv = 0;
+      cone += cv * cv;
+    }
+    out[0] = std::min(mn, 0.0);
+    out[1] = std::min(mv, 0.0);
+    out[2] = comp;
+    out[3] = std::sqrt(cone);
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C interface used by the tests (ctypes) and by bench.py's CPU baseline.
+extern "C" {
+
+void* or_create() { return new System(); }
+void or_destroy(void* s) { delete static_cast<System*>(s); }
+
+void or_set_geom_mode(int mode) { g_geomMode = mode; }
+int or_get_geom_mode() { return g_geomMode; }
+
+int or_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+// params: h, mu, tol, maxIt, gx, gy, gz, envelope, bins, warm, antirelax
+void or_set_params(void* sp, const double* a) {
+  System* s = static_cast<System*>(sp);
+  s->h = a[0]; s->mu = a[1]; s->tol = a[2]; s->maxIt = (int)a[3];
+  s->gravity = mk(a[4], a[5], a[6]);
+  s->envelope = a[7];
+  s->bins[0] = s->bins[1] = s->bins[2] = (unsigned)a[8];
+  s->warm = a[9] != 0;
+  s->antiRelax = a[10] != 0;
+}
+
+void or_set_bins(void* sp, int bx, int by, int bz) {
+  System* s = static_cast<System*>(sp);
+  s->bins[0] = bx; s->bins[1] = by; s->bins[2] = bz;
+}
+
+int or_add_bodies(void* sp, int n, const double* pos, const double* vel, const double* mass,
+                  const double* geom, const int* fixed, const int* family) {
+  System* s = static_cast<System*>(sp);
+  for (int i = 0; i < n; ++i) {
+    for (int c = 0; c < 3; ++c) {
+      s->p.push_back(pos[3 * i + c]);
+      s->v.push_back(vel[3 * i + c]);
+      s->geom.push_back(geom[3 * i + c]);
+    }
+    s->bodyMass.push_back(mass[i]);
+    s->fixed.push_back(fixed[i]);
+    s->family.push_back(family[i]);
+  }
+  return (int)s->fixed.size();
+}
+
+int or_add_bilateral(void* sp, int dofA, int dofB, double vel, double t0) {
+  System* s = static_cast<System*>(sp);
+  s->bilA.push_back(dofA);
+  s->bilB.push_back(dofB);
+  s->bilVel.push_back(vel);
+  s->bilT0.push_back(t0);
+  return (int)s->bilA.size();
+}
+
+int or_initialize(void* sp) {
+  static_cast<System*>(sp)->initialize();
+  return 0;
+}
+
+int or_step(void* sp) {
+  static_cast<System*>(sp)->step();
+  return 0;
+}
+
+// Run collision detection only (for pair-set checks).
+int or_collide(void* sp) {
+  System* s = static_cast<System*>(sp);
+  s->collide();
+  return s->nc;
+}
+
+int or_num_contacts(void* sp) { return static_cast<System*>(sp)->nc; }
+long or_num_possible(void* sp) { return (long)static_cast<System*>(sp)->numPossible; }
+int or_num_rows(void* sp) { return static_cast<System*>(sp)->rows(); }
+int or_iterations(void* sp) { return static_cast<System*>(sp)->iterations; }
+int or_backtracks(void* sp) { return static_cast<System*>(sp)->backtracks; }
+double or_residual(void* sp) { return static_cast<System*>(sp)->residual; }
+double or_objective(void* sp) { return static_cast<System*>(sp)->objective; }
+double or_time(void* sp) { return static_cast<System*>(sp)->time; }
+
+void or_get_contacts(void* sp, uint32_t* a, uint32_t* b, double* nap, double* loc) {
+  System* s = static_cast<System*>(sp);
+  for (int c = 0; c < s->nc; ++c) {
+    a[c] = s->idA[c];
+    b[c] = s->idB[c];
+    for (int q = 0; q < 4; ++q) nap[4 * c + q] = s->nAndP[4 * c + q];
+    if (loc)
+      for (int q = 0; q < 3; ++q) loc[3 * c + q] = s->loc[3 * c + q];
+  }
+}
+
+// which: 0 p, 1 v, 2 gamma, 3 f_contact, 4 r, 5 b, 6 k, 7 friction
+int or_get_vec(void* sp, int which, double* out, long cap) {
+  System* s = static_cast<System*>(sp);
+  const std::vector<double>* v = nullptr;
+  switch (which) {
+    case 0: v = &s->p; break;
+    case 1: v = &s->v; break;
+    case 2: v = &s->gamma; break;
+    case 3: v = &s->fContact; break;
+    case 4: v = &s->r; break;
+    case 5: v = &s->b; break;
+    case 6: v = &s->k; break;
+    case 7: v = &s->friction; break;
+    default: return -1;
+  }
+  long n = (long)v->size();
+  if (out) std::memcpy(out, v->data(), sizeof(double) * (size_t)std::min(n, cap));
+  return (int)n;
+}
+
+int or_set_vec(void* sp, int which, const double* in, long n) {
+  System* s = static_cast<System*>(sp);
+  std::vector<double>* v = nullptr;
+  switch (which) {
+    case 0: v = &s->p; break;
+    case 1: v = &s->v; break;
+    case 2: v = &s->gamma; break;
+    default: return -1;
+  }
+  v->assign(in, in + n);
+  return 0;
+}
+
+// D as COO (rows, cols, nnz returned; arrays filled when non-null)
+long or_get_D(void* sp, int* I, int* J, double* V, int* rows, int* cols) {
+  System* s = static_cast<System*>(sp);
+  *rows = s->D.rows;
+  *cols = s->D.cols;
+  if (I) {
+    std::memcpy(I, s->D.I.data(), sizeof(int) * s->D.I.size());
+    std::memcpy(J, s->D.J.data(), sizeof(int) * s->D.J.size());
+    std::memcpy(V, s->D.V.data(), sizeof(double) * s->D.V.size());
+  }
+  return (long)s->D.I.size();
+}
+
+// y = N x with the current D, M^-1 (APGD::performSchurComplementProduct)
+int or_schur(void* sp, const double* x, double* y) {
+  System* s = static_cast<System*>(sp);
+  std::vector<double> xi(x, x + s->rows()), yo;
+  s->schur(xi, yo);
+  std::memcpy(y, yo.data(), sizeof(double) * yo.size());
+  return 0;
+}
+
+// Build k, D, r for the current contacts without solving (for product checks).
+int or_assemble(void* sp) {
+  System* s = static_cast<System*>(sp);
+  s->collide();
+  s->buildK();
+  s->buildD();
+  s->buildR();
+  return s->nc;
+}
+
+// getResidual (APGD.cu:90-102) of an arbitrary gamma on the assembled system.
+double or_residual_of(void* sp, const double* g) {
+  System* s = static_cast<System*>(sp);
+  std::vector<double> x(g, g + s->rows()), t;
+  return s->getResidual(x, t);
+}
+
+// The per-row vector whose infinity norm getResidual returns.
+void or_residual_vector(void* sp, const double* g, double* out) {
+  System* s = static_cast<System*>(sp);
+  std::vector<double> x(g, g + s->rows()), t;
+  s->getResidual(x, t);
+  std::memcpy(out, t.data(), sizeof(double) * t.size());
+}
+
+// Cone projection of one 3-vector (APGD.cu:47-78).
+void or_project3(double* g, double mu) { System::projectOne(g, mu); }
+
+// Contact tangent frame from a normal (System.cu:753-762).
+void or_frame(const double* n, double* u, double* w) {
+  V3 uu, ww;
+  System::frame(n, &uu, &ww);
+  u[0] = uu.x; u[1] = uu.y; u[2] = uu.z;
+  w[0] = ww.x; w[1] = ww.y; w[2] = ww.z;
+}
+
+// Narrowphase on one pair of bodies: returns 1 if kept, with the stored
+// normal (B to A) and gap.
+int or_narrow_pair(void* sp, int a, int b, double* n, double* gap) {
+  System* s = static_cast<System*>(sp);
+  V3 l;
+  return s->narrow(a, b, n, gap, &l) ? 1 : 0;
+}
+
+void or_ccp_violation(void* sp, double* out) { static_cast<System*>(sp)->ccpViolation(out); }
+
+}  // extern "C"
