@@ -1,0 +1,200 @@
+"""Time-step throughput of the granular compaction workload on one MI355X.
+
+Workload: 112,000 spheres (radii uniform in [0.04, 0.06] m, seed 2002) settled
+in a 5 x 5 m footprint under a 50 kg lid driven down at 0.1 m/s by a bilateral
+row; mu = 0.3, h = 1e-3 s, APGD tolerance 1e-4 (about 256k contacts, D of
+768k x 336k).  One step is System::DoTimeStep (System.cu:544-602): collision
+detection, assembly, APGD to the tolerance, update.
+
+The headline `value` is time-steps/s of the mixed-precision solver (FP32
+D, M^-1 and products, FP64 iterates, tests and update).  The same run times
+the FP64 graph-driven solver and the plain FP64 solver (one kernel per
+vector operation, host-driven like the reference's CUSP calls) from the same
+initial state, and the CPU oracle on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S] [--no-cpu]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, _HERE)
+
+HBM_PEAK_GBS = 8000.0
+TOL = 1e-4
+
+
+def _hip():
+    lib = ctypes.CDLL("libamdhip64.so")
+    lib.hipDeviceSynchronize.restype = ctypes.c_int
+    return lib
+
+
+def p1_bytes_per_contact(mixed: bool) -> int:
+    """Algorithmic bytes the contact pass P1 moves per contact (3 rows).
+
+    Reads gamma_k, gamma_{k-1} (FP64), N gamma_k, N gamma_{k-1} and r (TM),
+    the stored frame (FP64 n, or FP32 n and u); writes gamma_new (FP64), the
+    world-frame impulse J = F^T gamma_new (3 TM) and, in the mixed solver,
+    delta = gamma_new - y (3 TM).
+    """
+    s = 4 if mixed else 8
+    return 2 * 24 + 9 * s + 24 + 24 + 3 * s + (3 * s if mixed else 0)
+
+
+def run_gpu(scene, solver, steps, warmup, profile_kernel=0):
+    from dvide_amd import system
+
+    g = system.from_scene(scene, solver, tol=TOL)
+    hip = _hip()
+    try:
+        for _ in range(warmup):
+            g.step()
+        g.setProfileKernel(profile_kernel)
+        hip.hipDeviceSynchronize()
+        lat, its, res, nc, passes, kms, kcnt = [], [], [], [], [], 0.0, 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ts = time.perf_counter()
+            g.step()
+            lat.append(time.perf_counter() - ts)
+            st = g.stats()
+            its.append(st.iterations)
+            res.append(st.residual)
+            nc.append(st.num_contacts)
+            passes.append(st.passes)
+            kms += st.prof_kernel_ms
+            kcnt += st.prof_kernel_count
+        hip.hipDeviceSynchronize()
+        elapsed = time.perf_counter() - t0
+        last = g.stats()
+        return dict(
+            elapsed=elapsed, lat=np.array(lat), iterations=np.array(its), residual=np.array(res),
+            contacts=np.array(nc), passes=np.array(passes), kernel_ms=kms, kernel_count=kcnt,
+            phases=dict(collision=last.ms_collision, assembly=last.ms_assembly, solve=last.ms_solve,
+                        update=last.ms_update),
+            nb=g.numBodies, v=g.v,
+        )
+    finally:
+        g.close()
+
+
+def cpu_baseline(scene, steps):
+    """The oracle (FP64 C++/OpenMP restatement) on the host cores."""
+    import oracle
+
+    cores = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    oracle.set_threads(cores)
+    o = oracle.OracleSystem(scene, tol=TOL)
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step()
+        its.append(o.iterations)
+    dt = time.perf_counter() - t0
+    return dict(value=steps / dt, unit="time-steps/s", cores=cores, kind="port",
+                sample=f"first {steps} time steps of the same workload, APGD to tol {TOL} "
+                       f"({'/'.join(map(str, its))} iterations), FP64, {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the contact time step is not sharded across GPUs "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    from dvide_amd import scenes, system
+
+    scene = scenes.compaction(2002)
+    mixed = run_gpu(scene, system.SOLVER_MIXED, args.steps, args.warmup, profile_kernel=system.PROFILE_P1)
+    others = {}
+    if not args.only_mixed:
+        others["fp64"] = run_gpu(scene, system.SOLVER_FUSED, args.steps, args.warmup)
+        others["plain_fp64"] = run_gpu(scene, system.SOLVER_PLAIN, max(2, args.steps // 4), args.warmup and 1)
+
+    value = args.steps / mixed["elapsed"]
+    nc = int(np.mean(mixed["contacts"]))
+    k_us = 1e3 * mixed["kernel_ms"] / max(mixed["kernel_count"], 1)
+    alg = p1_bytes_per_contact(True) * nc
+    achieved = alg / (k_us * 1e-6) / 1e9 if k_us > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(_HERE, "profiles", "p1_traffic_compaction.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("mixed_bytes_per_launch")
+
+    def lat_ms(r):
+        return dict(p50=round(1e3 * float(np.percentile(r["lat"], 50)), 3),
+                    p99=round(1e3 * float(np.percentile(r["lat"], 99)), 3))
+
+    line = {
+        "metric": "time-steps/s at 250k contacts (res<=1e-4); p50/p99 solve latency, mixed vs FP64",
+        "value": round(value, 3),
+        "unit": "time-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * mixed["elapsed"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded scene generator)",
+        "config": {
+            "workload": "granular compaction: 112k spheres r~U[0.04,0.06] m, 5x5 m footprint, "
+                        "lid at 0.1 m/s, mu 0.3, h 1e-3, APGD tol 1e-4",
+            "contacts": nc,
+            "bodies": mixed["nb"],
+            "solver": "APGD mixed (FP32 D/M^-1/products, FP64 iterates/tests/update)",
+        },
+        "latency_ms": lat_ms(mixed),
+        "iterations_mean": round(float(np.mean(mixed["iterations"])), 1),
+        "residual_max": float(np.max(mixed["residual"])),
+        "phases_ms_last_step": {k: round(v, 3) for k, v in mixed["phases"].items()},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "kP1<float> (contact pass: momentum, projection, F^T gamma)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "avg_us": round(k_us, 3),
+            "launches_timed": mixed["kernel_count"],
+            "bytes_per_contact": p1_bytes_per_contact(True),
+        },
+    }
+    for name, r in others.items():
+        k = len(r["lat"])
+        line[name] = {
+            "value": round(k / r["elapsed"], 3),
+            "ms_per_step": round(1e3 * r["elapsed"] / k, 3),
+            "latency_ms": lat_ms(r),
+            "iterations_mean": round(float(np.mean(r["iterations"])), 1),
+            "residual_max": float(np.max(r["residual"])),
+        }
+    if "plain_fp64" in line:
+        line["mixed_vs_plain_fp64"] = round(value / line["plain_fp64"]["value"], 2)
+    if "fp64" in line:
+        line["mixed_vs_fp64"] = round(value / line["fp64"]["value"], 2)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(scene, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -96,9 +96,10 @@ struct GatherArgs {
 // skipFixed, bodies with M^-1 = 0 get tmp = +0 without the sum (0.0 + 0*f).
 __global__ __launch_bounds__(kT) void kGatherDT(GatherArgs a, const double* __restrict__ x,
                                                 double* __restrict__ fOut, double* __restrict__ tmpOut,
-                                                int skipFixed) {
+                                                int skipFixed, int skipHeavy) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.nb) return;
+  if (skipHeavy && a.iStart[g + 1] - a.iStart[g] > kHeavyDegree) return;
   const double mi = a.minv[g];
   if (skipFixed && mi == 0) {
     tmpOut[3 * g] = 0.0; tmpOut[3 * g + 1] = 0.0; tmpOut[3 * g + 2] = 0.0;
@@ -137,6 +138,57 @@ __global__ __launch_bounds__(kT) void kGatherDT(GatherArgs a, const double* __re
     double t0 = 0.0, t1 = 0.0, t2 = 0.0;
     t0 = t0 + mi * s0; t1 = t1 + mi * s1; t2 = t2 + mi * s2;
     tmpOut[3 * g] = t0; tmpOut[3 * g + 1] = t1; tmpOut[3 * g + 2] = t2;
+  }
+}
+
+// f = D^T x for the bodies of the heavy list (more than kHeavyDegree
+// incidences), one workgroup each, FP64 sums in a fixed tree order; used by
+// the update of the persistent solvers in place of the sequential sums.
+__global__ __launch_bounds__(kT) void kGatherHeavy(GatherArgs a, const int* heavy, const int* counters,
+                                                   const double* __restrict__ x, double* __restrict__ fOut) {
+  const int nHeavy = counters[2];
+  __shared__ double red[kT / 64][3];
+  const long S = a.cap;
+  for (int hq = blockIdx.x; hq < nHeavy; hq += gridDim.x) {
+    const int g = heavy[hq];
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int k = a.iStart[g] + threadIdx.x; k < a.iStart[g + 1]; k += kT) {
+      const int e = a.inc[k];
+      const int c = e >> 1;
+      const double sg = (e & 1) ? -1.0 : 1.0;
+      const long base = a.nbil + 3L * c;
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr) {
+        const double xr = sg * x[base + rr];
+        s0 += a.frame[(3 * rr + 0) * S + c] * xr;
+        s1 += a.frame[(3 * rr + 1) * S + c] * xr;
+        s2 += a.frame[(3 * rr + 2) * S + c] * xr;
+      }
+    }
+    if (threadIdx.x == 0) {
+      for (int k = a.bilIncStart[g]; k < a.bilIncStart[g + 1]; ++k) {
+        const int e = a.bilInc[k];
+        const double t = ((e & 1) ? -1.0 : 1.0) * x[e >> 3];
+        const int comp = (e >> 1) & 3;
+        if (comp == 0) s0 += t; else if (comp == 1) s1 += t; else s2 += t;
+      }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      s0 += __shfl_xor(s0, d, 64);
+      s1 += __shfl_xor(s1, d, 64);
+      s2 += __shfl_xor(s2, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      red[threadIdx.x >> 6][0] = s0; red[threadIdx.x >> 6][1] = s1; red[threadIdx.x >> 6][2] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      double v = 0;
+      for (int w = 0; w < kT / 64; ++w) v += red[w][threadIdx.x];
+      fOut[3 * g + threadIdx.x] = v;
+    }
+    __syncthreads();
   }
 }
 
@@ -245,7 +297,7 @@ void buildR(Scene& s) {
 
 void gatherExact(Scene& s, const double* x, double* fOut, double* tmpOut, bool skipFixed) {
   hipLaunchKernelGGL(kGatherDT, dim3(blocksFor(s.nb, kT)), dim3(kT), 0, s.stream, gatherArgs(s), x, fOut, tmpOut,
-                     skipFixed ? 1 : 0);
+                     skipFixed ? 1 : 0, 0);
 }
 
 // y = D (M^-1 (D^T x))  (APGD.cu:81-87)
@@ -263,8 +315,17 @@ void rowsD(Scene& s, const double* x, const double* badd, double* y) {
     hipLaunchKernelGGL(kRowsD, dim3(blocksFor(items, kT)), dim3(kT), 0, s.stream, rowArgs(s), x, badd, y);
 }
 
-void updateState(Scene& s, bool haveRows) {
-  if (haveRows) gatherExact(s, s.gamma.p, s.fContact.p, nullptr, false);
+void updateState(Scene& s, bool haveRows, bool exactOrder) {
+  if (haveRows) {
+    if (exactOrder) {
+      gatherExact(s, s.gamma.p, s.fContact.p, nullptr, false);
+    } else {
+      hipLaunchKernelGGL(kGatherDT, dim3(blocksFor(s.nb, kT)), dim3(kT), 0, s.stream, gatherArgs(s),
+                         (const double*)s.gamma.p, s.fContact.p, (double*)nullptr, 0, 1);
+      hipLaunchKernelGGL(kGatherHeavy, dim3(64), dim3(kT), 0, s.stream, gatherArgs(s), (const int*)s.heavy.p,
+                         (const int*)s.counters.p, (const double*)s.gamma.p, s.fContact.p);
+    }
+  }
   hipLaunchKernelGGL(kUpdate, dim3(blocksFor(3L * s.nb, kT)), dim3(kT), 0, s.stream, s.nb, s.minv.p, s.k.p,
                      s.fContact.p, s.v.p, s.p.p, s.h, haveRows ? 1 : 0);
 }

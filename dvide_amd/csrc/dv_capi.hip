@@ -145,7 +145,7 @@ void step(Scene& s) {
   DV_HIP(hipEventRecord(s.ev[2], st));
   if (rows) runSolver(s);
   DV_HIP(hipEventRecord(s.ev[3], st));
-  updateState(s, rows);
+  updateState(s, rows, s.solver == DV_SOLVER_APGD_PLAIN);
   DV_HIP(hipEventRecord(s.ev[4], st));
   DV_HIP(hipEventSynchronize(s.ev[4]));
   DV_HIP(hipGetLastError());
@@ -195,19 +195,23 @@ int dv_scene_destroy(dv_scene* h) {
   for (auto* b : dbufs) b->release();
   dv::DevBuf<int>* ibufs[] = {&s.family, &s.smallIds, &s.bigIds, &s.bilA, &s.bilB, &s.bilIncStart,
                               &s.bilInc, &s.cellItems, &s.cntUp, &s.cntDown, &s.cStart, &s.iStart,
-                              &s.scanTmp, &s.counters, &s.hostTotals, &s.cA, &s.cB, &s.inc, &s.heavy};
+                              &s.scanTmp, &s.counters, &s.hostTotals, &s.cA, &s.cB, &s.inc, &s.heavy,
+                              &s.bigChunks};
   for (auto* b : ibufs) b->release();
   s.slotOf.release(); s.slotCount.release(); s.slotStart.release(); s.slotFill.release();
   s.frame32.release(); s.work32.release();
   s.graphFp64.reset();
   s.graphMixed.reset();
   if (s.state) (void)hipFree(s.state);
+  if (s.ticket) (void)hipFree(s.ticket);
+  s.workM64.release();
   if (s.solveDims) (void)hipFree(s.solveDims);
   if (s.stateHost) (void)hipHostFree(s.stateHost);
   if (s.dimsHost) (void)hipHostFree(s.dimsHost);
   if (s.pinned) (void)hipHostFree(s.pinned);
   for (auto& e : s.ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : s.profEv) (void)hipEventDestroy(e);
   if (s.stream) (void)hipStreamDestroy(s.stream);
   delete h;
   return DV_OK;
@@ -298,6 +302,10 @@ int dv_set_param(dv_scene* h, int param, double v) {
       case DV_PARAM_TIME: s.time = v; break;
       case DV_PARAM_VERBOSE: s.verbose = (int)v; break;
       case DV_PARAM_GRID: s.grid = std::max(0, (int)v); break;
+      case DV_PARAM_PROFILE:
+        if (v < 0 || v > 4) return dv::fail(s, DV_ERR_ARG, "profile kernel is 0..4");
+        s.profKernel = (int)v;
+        break;
       default: return dv::fail(s, DV_ERR_ARG, "unknown parameter");
     }
     return DV_OK;
@@ -321,6 +329,7 @@ int dv_get_param(const dv_scene* hc, int param, double* v) {
       case DV_PARAM_TIME: *v = s.time; break;
       case DV_PARAM_VERBOSE: *v = s.verbose; break;
       case DV_PARAM_GRID: *v = s.grid; break;
+      case DV_PARAM_PROFILE: *v = s.profKernel; break;
       default: return dv::fail(s, DV_ERR_ARG, "unknown parameter");
     }
     return DV_OK;

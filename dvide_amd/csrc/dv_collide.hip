@@ -42,7 +42,7 @@ struct CdArgs {
   const int* smallIds;
   const int* bigIds;
   int nb, nsmall, nbig;
-  double cs, envelope;
+  double cs, envelope, rmaxSmall;
   unsigned mask;
   int mode;
   const unsigned* slotStart;
@@ -76,6 +76,9 @@ __device__ __forceinline__ bool keepPair(const CdArgs& a, int x, int y) {
 
 // Visit every candidate partner j != g of small geometry g: the occupants of
 // the (deduplicated) hash slots of its 27 neighbour cells, then the big list.
+// Cells have edge cs = 2 (rmax + envelope) >= r_g + r_j + envelope, the
+// largest centre distance of a kept pair, so a kept partner lies in one of
+// the 27 cells around g's cell.
 template <class F>
 __device__ __forceinline__ void forEachCandidate(const CdArgs& a, int g, F&& fn) {
   d3 pg = ldPos(a.p, g);
@@ -151,11 +154,17 @@ __device__ int blockSum(int v) {
   return s;
 }
 
-// per big geometry (one workgroup each): partners among all geometries
-__global__ __launch_bounds__(kT) void countBig(CdArgs a, int* cntUp, int* cntDown) {
-  int b = a.bigIds[blockIdx.x];
+// Big geometries (boxes, large spheres) are paired with every geometry in
+// chunks of kChunk partners: block (chunk, b) counts the kept partners of big
+// geometry b in its chunk; a per-big scan over the chunks then gives each
+// block its output offsets, so the emit pass writes the row in order.
+constexpr int kChunk = 2048;
+
+__global__ __launch_bounds__(kT) void countBigChunks(CdArgs a, int nchunks, int* chUp, int* chDown) {
+  const int b = a.bigIds[blockIdx.y];
+  const int j0 = blockIdx.x * kChunk, j1 = min(a.nb, j0 + kChunk);
   int up = 0, down = 0;
-  for (int j = threadIdx.x; j < a.nb; j += kT) {
+  for (int j = j0 + threadIdx.x; j < j1; j += kT) {
     if (j == b) continue;
     if (keepPair(a, b, j)) {
       if (j > b) ++up; else ++down;
@@ -164,9 +173,25 @@ __global__ __launch_bounds__(kT) void countBig(CdArgs a, int* cntUp, int* cntDow
   up = blockSum(up);
   down = blockSum(down);
   if (threadIdx.x == 0) {
-    cntUp[b] = up;
-    cntDown[b] = down;
+    chUp[blockIdx.y * nchunks + blockIdx.x] = up;
+    chDown[blockIdx.y * nchunks + blockIdx.x] = down;
   }
+}
+
+// per big geometry: exclusive scan of its chunk counts (in place) and totals
+__global__ void scanBigChunks(CdArgs a, int nchunks, int* chUp, int* chDown, int* cntUp, int* cntDown) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.nbig) return;
+  int su = 0, sd = 0;
+  for (int k = 0; k < nchunks; ++k) {
+    const int u = chUp[q * nchunks + k], d = chDown[q * nchunks + k];
+    chUp[q * nchunks + k] = su;
+    chDown[q * nchunks + k] = sd;
+    su += u;
+    sd += d;
+  }
+  cntUp[a.bigIds[q]] = su;
+  cntDown[a.bigIds[q]] = sd;
 }
 
 __global__ __launch_bounds__(kT) void addCounts(const int* cntUp, const int* cntDown, int* tot, int n) {
@@ -233,22 +258,26 @@ __global__ __launch_bounds__(kT) void emitSmall(CdArgs a, EmitOut o) {
   for (int q = 0; q < up; ++q) writeRecord(a, o, c0 + q, g, o.cB[c0 + q]);
 }
 
-// big geometry rows: ordered stream compaction over all partners
-__global__ __launch_bounds__(kT) void emitBig(CdArgs a, EmitOut o) {
-  const int b = a.bigIds[blockIdx.x];
-  const int c0 = o.cStart[b], i0 = o.iStart[b];
+// big geometry rows: ordered stream compaction of each chunk's partners
+__global__ __launch_bounds__(kT) void emitBigChunks(CdArgs a, EmitOut o, int nchunks, const int* chUp,
+                                                    const int* chDown) {
+  const int b = a.bigIds[blockIdx.y];
+  const int j0 = blockIdx.x * kChunk, j1 = min(a.nb, j0 + kChunk);
   __shared__ int baseUp, baseDown;
-  if (threadIdx.x == 0) { baseUp = 0; baseDown = 0; }
+  if (threadIdx.x == 0) {
+    baseUp = o.cStart[b] + chUp[blockIdx.y * nchunks + blockIdx.x];
+    baseDown = o.iStart[b] + chDown[blockIdx.y * nchunks + blockIdx.x];
+  }
   __syncthreads();
   __shared__ int wUp[kT / 64], wDown[kT / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j0 = 0; j0 < a.nb; j0 += kT) {
-    int j = j0 + threadIdx.x;
-    bool kept = (j < a.nb && j != b) ? keepPair(a, b, j) : false;
-    bool isUp = kept && j > b, isDown = kept && j < b;
-    unsigned long long mUp = __ballot(isUp), mDown = __ballot(isDown);
-    unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int rUp = __popcll(mUp & below), rDown = __popcll(mDown & below);
+  for (int jb = j0; jb < j1; jb += kT) {
+    const int j = jb + threadIdx.x;
+    const bool kept = (j < j1 && j != b) ? keepPair(a, b, j) : false;
+    const bool isUp = kept && j > b, isDown = kept && j < b;
+    const unsigned long long mUp = __ballot(isUp), mDown = __ballot(isDown);
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int rUp = __popcll(mUp & below), rDown = __popcll(mDown & below);
     if (lane == 0) { wUp[wave] = __popcll(mUp); wDown[wave] = __popcll(mDown); }
     __syncthreads();
     int offUp = baseUp, offDown = baseDown, totUp = 0, totDown = 0;
@@ -258,8 +287,8 @@ __global__ __launch_bounds__(kT) void emitBig(CdArgs a, EmitOut o) {
       totUp += wUp[w];
       totDown += wDown[w];
     }
-    if (isUp) writeRecord(a, o, c0 + offUp + rUp, b, j);
-    if (isDown) o.inc[i0 + offDown + rDown] = j;
+    if (isUp) writeRecord(a, o, offUp + rUp, b, j);
+    if (isDown) o.inc[offDown + rDown] = j;
     __syncthreads();
     if (threadIdx.x == 0) { baseUp += totUp; baseDown += totDown; }
     __syncthreads();
@@ -356,6 +385,7 @@ void collide(Scene& s) {
   a.nsmall = s.nsmall;
   a.nbig = s.nbig;
   a.cs = 2.0 * (s.cellSize + s.envelope);
+  a.rmaxSmall = s.cellSize;
   if (!(a.cs > 0)) a.cs = 1.0;
   a.envelope = s.envelope;
   a.mask = s.hashSize - 1;
@@ -374,7 +404,15 @@ void collide(Scene& s) {
                        s.slotFill.p, s.cellItems.p);
     hipLaunchKernelGGL(countSmall, dim3(blocksFor(s.nsmall, kT)), dim3(kT), 0, st, a, s.cntUp.p, s.cntDown.p);
   }
-  if (s.nbig) hipLaunchKernelGGL(countBig, dim3(s.nbig), dim3(kT), 0, st, a, s.cntUp.p, s.cntDown.p);
+  const int nchunks = blocksFor(s.nb, kChunk);
+  s.bigChunks.reserve((size_t)2 * std::max(1, s.nbig * nchunks));
+  int* chUp = s.bigChunks.p;
+  int* chDown = s.bigChunks.p + (size_t)std::max(1, s.nbig * nchunks);
+  if (s.nbig) {
+    hipLaunchKernelGGL(countBigChunks, dim3(nchunks, s.nbig), dim3(kT), 0, st, a, nchunks, chUp, chDown);
+    hipLaunchKernelGGL(scanBigChunks, dim3(blocksFor(s.nbig, 64)), dim3(64), 0, st, a, nchunks, chUp, chDown,
+                       s.cntUp.p, s.cntDown.p);
+  }
   // contact rows and incidence offsets
   s.scanTmp.reserve(16);
   exclusiveScan(s, s.cntUp.p, s.cStart.p, s.nb);
@@ -406,7 +444,8 @@ void collide(Scene& s) {
   o.inc = s.inc.p;
   o.cStart = s.cStart.p; o.iStart = s.iStart.p; o.cntUp = s.cntUp.p; o.cntDown = s.cntDown.p;
   if (s.nsmall) hipLaunchKernelGGL(emitSmall, dim3(blocksFor(s.nsmall, kT)), dim3(kT), 0, st, a, o);
-  if (s.nbig) hipLaunchKernelGGL(emitBig, dim3(s.nbig), dim3(kT), 0, st, a, o);
+  if (s.nbig)
+    hipLaunchKernelGGL(emitBigChunks, dim3(nchunks, s.nbig), dim3(kT), 0, st, a, o, nchunks, chUp, chDown);
   hipLaunchKernelGGL(buildIncidence, dim3(blocksFor(s.nb, kT)), dim3(kT), 0, st, s.nb, s.cStart.p,
                      s.iStart.p, s.cntUp.p, s.cntDown.p, s.cB.p, s.inc.p, s.heavy.p, s.counters.p);
   DV_HIP(hipGetLastError());

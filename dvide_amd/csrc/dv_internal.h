@@ -94,6 +94,8 @@ struct Scene {
   int verbose = 0;
   int grid = 0;
   bool noGraph = false;  // launch the solver passes directly (profiling)
+  int profKernel = 0;    // DV_PARAM_PROFILE
+  std::vector<hipEvent_t> profEv;
   double time = 0;
   int timeIndex = 0;
   bool initialized = false;
@@ -138,15 +140,18 @@ struct Scene {
   DevBuf<float> frame32; // FP32 copy for the mixed solver
   DevBuf<int> inc;       // per-body incidences (contact << 1 | side)
   DevBuf<int> heavy;     // bodies processed by a whole workgroup
+  DevBuf<int> bigChunks; // per (big geometry, chunk) partner counts
 
   // --- system of the step (flat rows: nbil bilateral, then 3 per contact)
   int R = 0, gammaLen = 0;
   DevBuf<double> r, b, gamma;
   // fused solver work
-  DevBuf<double> work;   // ring[3], ringN[3], J, gHat, tmpS  (FP64)
-  DevBuf<float> work32;  // same for the mixed solver
+  DevBuf<double> work;   // solver iterates (FP64 in every solver)
+  DevBuf<double> workM64;  // products of the FP64 solver
+  DevBuf<float> work32;  // products and matrix data of the mixed solver
   DevBuf<double> partials;
   ApgdState* state = nullptr;      // device-resident APGD state
+  unsigned* ticket = nullptr;      // last-block counter of the solver passes
   SolveDims* solveDims = nullptr;
   ApgdState* stateHost = nullptr;  // pinned
   SolveDims* dimsHost = nullptr;   // pinned
@@ -164,7 +169,7 @@ void collide(Scene& s);
 // assembly of k and r, update (dv_assemble.hip)
 void buildK(Scene& s);
 void buildR(Scene& s);
-void updateState(Scene& s, bool haveRows);
+void updateState(Scene& s, bool haveRows, bool exactOrder);
 void schurExact(Scene& s, const double* x, double* y);  // device pointers, flat R
 void gatherExact(Scene& s, const double* x, double* fOut, double* tmpOut, bool skipFixed);
 void ensureRowBuffers(Scene& s);

@@ -22,7 +22,9 @@ LIB_PATH = os.path.join(_HERE, "libdvide_hip.so")
 DV_OK = 0
 SOLVER_FUSED, SOLVER_MIXED, SOLVER_PLAIN = 1, 2, 3
 (PARAM_H, PARAM_MU, PARAM_TOL, PARAM_MAX_ITER, PARAM_GX, PARAM_GY, PARAM_GZ, PARAM_ENVELOPE,
- PARAM_WARM_START, PARAM_SOLVER, PARAM_GEOM_MODE, PARAM_TIME, PARAM_VERBOSE, PARAM_GRID) = range(14)
+ PARAM_WARM_START, PARAM_SOLVER, PARAM_GEOM_MODE, PARAM_TIME, PARAM_VERBOSE, PARAM_GRID,
+ PARAM_PROFILE) = range(15)
+PROFILE_OFF, PROFILE_P1, PROFILE_P2, PROFILE_P3, PROFILE_DECIDE = range(5)
 VEC_P, VEC_V, VEC_GAMMA, VEC_F_CONTACT, VEC_R, VEC_K, VEC_B = range(7)
 
 EXPORTED = (
@@ -42,6 +44,8 @@ class DvStats(ctypes.Structure):
         ("ms_total", ctypes.c_double), ("ms_collision", ctypes.c_double),
         ("ms_assembly", ctypes.c_double), ("ms_solve", ctypes.c_double),
         ("ms_update", ctypes.c_double), ("time", ctypes.c_double),
+        ("prof_kernel_ms", ctypes.c_double), ("prof_kernel_count", ctypes.c_int),
+        ("passes", ctypes.c_int),
     ]
 
 
@@ -235,6 +239,10 @@ class System:
     def setGrid(self, g):
         self._set(PARAM_GRID, g)
 
+    def setProfileKernel(self, k):
+        """Time one solver kernel per pass with HIP events (PROFILE_*)."""
+        self._set(PARAM_PROFILE, k)
+
     def add(self, body: Body) -> int:
         self.bodies.append(body)
         return len(self.bodies)
@@ -296,6 +304,12 @@ class System:
     def numCollisions(self):
         n = ctypes.c_int()
         self._check(self._L.dv_num_contacts(self._h, ctypes.byref(n)))
+        return n.value
+
+    @property
+    def numBodies(self):
+        n = ctypes.c_int()
+        self._check(self._L.dv_num_bodies(self._h, ctypes.byref(n)))
         return n.value
 
     def vector(self, which):

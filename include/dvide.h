@@ -43,9 +43,9 @@ typedef struct dv_scene dv_scene;
 
 /* solver implementations (the reference's factory, System.cu:77-101, knows
  * 1=APGD .. 7=PJKIP; only APGD is on this path, in three builds) */
-#define DV_SOLVER_APGD_FUSED 1 /* persistent cooperative kernel, FP64      */
-#define DV_SOLVER_APGD_MIXED 2 /* persistent kernel, FP32 storage, FP64 tests */
-#define DV_SOLVER_APGD_PLAIN 3 /* host-driven FP64, one kernel per CUSP op  */
+#define DV_SOLVER_APGD_FUSED 1 /* device-resident state machine in a hipGraph, FP64 */
+#define DV_SOLVER_APGD_MIXED 2 /* same, FP32 D/M^-1/products, FP64 iterates and tests */
+#define DV_SOLVER_APGD_PLAIN 3 /* host-driven FP64, one kernel per CUSP operation */
 
 /* double parameters (dv_set_param / dv_get_param) */
 #define DV_PARAM_H 0          /* time step              System::setTimeStep, System.cu:135 */
@@ -63,7 +63,13 @@ typedef struct dv_scene dv_scene;
                                  (helper_math_double.h:576-598); 0: plain FP64 */
 #define DV_PARAM_TIME 11      /* System::time (read/write) */
 #define DV_PARAM_VERBOSE 12   /* Solver::verbose */
-#define DV_PARAM_GRID 13      /* workgroups of the persistent solver (0 = auto) */
+#define DV_PARAM_GRID 13      /* unused (accepted for compatibility) */
+#define DV_PARAM_PROFILE 14   /* time one kernel of the graph-driven solvers with HIP
+                                 events: 0 off, 1 P1 (contacts), 2 P2 (bodies),
+                                 3 P3 (contacts), 4 the decide kernel.  The first 16
+                                 passes of each solve are then launched on the scene's
+                                 stream with events around that kernel (the sample);
+                                 the rest run in the captured graph */
 
 /* state vectors (dv_get_vector / dv_set_vector) */
 #define DV_VEC_P 0         /* System::p_d        3*nb */
@@ -88,6 +94,10 @@ typedef struct dv_stats {
   double ms_solve;
   double ms_update;
   double time;            /* System::time after the step */
+  double prof_kernel_ms;  /* DV_PARAM_PROFILE: summed duration of the timed kernel
+                             over the sampled passes */
+  int prof_kernel_count;  /* ... and the number of those launches (passes that ran) */
+  int passes;             /* solver passes (Schur products) of the step */
 } dv_stats;
 
 /* --- scene lifetime ------------------------------------------------------ */
