@@ -170,8 +170,20 @@ using dv::Scene;
 
 extern "C" {
 
+namespace {
+thread_local std::string g_createError;  // dv_last_error(NULL) after a failed create
+}
+
 int dv_scene_create(dv_scene** out, int solver) {
   if (!out) return DV_ERR_ARG;
+  *out = nullptr;
+  // no CPU fallback: without a HIP device there is nothing to run on
+  int devices = 0;
+  const hipError_t e = hipGetDeviceCount(&devices);
+  if (e != hipSuccess || devices == 0) {
+    g_createError = std::string("no HIP device visible (") + hipGetErrorString(e) + ")";
+    return DV_ERR_HIP;
+  }
   try {
     dv_scene* h = new dv_scene();
     const char* ng = getenv("DVIDE_NO_GRAPH");
@@ -217,7 +229,7 @@ int dv_scene_destroy(dv_scene* h) {
   return DV_OK;
 }
 
-const char* dv_last_error(const dv_scene* h) { return h ? h->s.err.c_str() : "null scene"; }
+const char* dv_last_error(const dv_scene* h) { return h ? h->s.err.c_str() : g_createError.c_str(); }
 
 int dv_add_bodies(dv_scene* h, int n, const double* pos, const double* vel, const double* mass,
                   const double* geom, const int* fixed, const int* family) {

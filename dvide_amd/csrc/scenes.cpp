@@ -224,15 +224,24 @@ Scene* latticeBox(uint64_t seed, int perSide) {
 }
 
 // Granular compaction: n spheres r ~ U[0.04,0.06], mu = 0.3, settled in a
-// W x W open box and capped by a lid box driven down at 0.1 m/s through a
-// bilateral DOF row on its y coordinate (System.cu:522-532); the lid's x and z
-// DOFs are held by two more bilateral rows.
-Scene* compaction(uint64_t seed, int n, double W, int sweeps) {
+// W x W open box and capped by a rigid lid box moving down at 0.1 m/s.
+//   lidMode 0: a free lid as heavy as the spheres together, pressing with its
+//     weight (models/test_compaction.cu:272-277), released at -0.1 m/s, with
+//     APGD warm starting as that driver sets it (test_compaction.cu:245-247);
+//   lidMode 1: a 50 kg lid driven at -0.1 m/s through a bilateral DOF row on
+//     its y coordinate (System.cu:522-532), its x and z held by two more rows.
+//     Driving a rigid lid at a fixed speed into a jammed packing of rigid
+//     spheres has no solution once the lid bears on the packing, so APGD
+//     stalls from the third step on; this mode serves the bilateral rows'
+//     tests on the first steps.
+Scene* compaction(uint64_t seed, int n, double W, int sweeps, int lidMode, double lidMass,
+                  double lidSpeed) {
   Scene* s = new Scene();
   defaultParams(*s);
   s->params[P_MU] = 0.3;
   s->params[P_MAXIT] = 1000;
   s->params[P_STEPS] = 3;
+  s->params[P_WARM] = lidMode == 0 ? 1 : 0;
   Rng rng(seed);
   std::vector<double> x, r;
   deposit(rng, n, 0.04, 0.06, W, W, x, r);
@@ -244,9 +253,18 @@ Scene* compaction(uint64_t seed, int n, double W, int sweeps) {
   // lid bottom face at the highest sphere top: the top layer is in contact
   int lid = s->addBox(0.5 * W, top + lt, 0.5 * W, 0.5 * W - 0.001, lt, 0.5 * W - 0.001, false,
                       50.0);
-  for (int i = 0; i < n; ++i) s->addSphere(x[3 * i], x[3 * i + 1], x[3 * i + 2], r[i], 2500.0);
-  s->bil_dofs.insert(s->bil_dofs.end(), {3 * lid + 0, -1, 3 * lid + 1, -1, 3 * lid + 2, -1});
-  s->bil_info.insert(s->bil_info.end(), {0.0, 0.0, -0.1, 0.0, 0.0, 0.0});
+  double total = 0;
+  for (int i = 0; i < n; ++i) {
+    const int b = s->addSphere(x[3 * i], x[3 * i + 1], x[3 * i + 2], r[i], 2500.0);
+    total += s->mass[b];
+  }
+  if (lidMode == 0) {
+    s->mass[lid] = lidMass > 0 ? lidMass : total;
+    s->vel[3 * lid + 1] = -lidSpeed;
+  } else {
+    s->bil_dofs.insert(s->bil_dofs.end(), {3 * lid + 0, -1, 3 * lid + 1, -1, 3 * lid + 2, -1});
+    s->bil_info.insert(s->bil_info.end(), {0.0, 0.0, -0.1, 0.0, 0.0, 0.0});
+  }
   return s;
 }
 
@@ -318,7 +336,10 @@ enum {
 // Build a scene.  `a` holds kind-specific arguments:
 //   BALL_ON_BOX: a[0] = lateral gravity g_x, a[1] = h
 //   LATTICE_BOX: a[0] = spheres per side (default 10)
-//   COMPACTION:  a[0] = sphere count, a[1] = footprint width, a[2] = settle sweeps
+//   COMPACTION:  a[0] = sphere count, a[1] = footprint width, a[2] = settle sweeps,
+//                a[3] = lid mode (0 free heavy lid, 1 bilateral-driven lid),
+//                a[4] = free lid mass (0: the spheres' total), a[5] = free lid
+//                speed (0: 0.1 m/s, negative: at rest)
 //   PILE:        a[0] = target contacts, a[1] = rmin, a[2] = rmax, a[3] = mu, a[4] = sweeps
 //   CLOUD:       a[0] = sphere count, a[1] = cube edge, a[2] = rmin, a[3] = rmax
 //   FREE_FALL:   a[0] = sphere count
@@ -329,7 +350,7 @@ void* dvs_generate(int kind, uint64_t seed, const double* a) {
     case DVS_LATTICE_BOX: return latticeBox(seed, a[0] > 0 ? (int)a[0] : 10);
     case DVS_COMPACTION:
       return compaction(seed, a[0] > 0 ? (int)a[0] : 64000, a[1] > 0 ? a[1] : 4.0,
-                        a[2] > 0 ? (int)a[2] : 400);
+                        a[2] > 0 ? (int)a[2] : 400, (int)a[3], a[4], a[5] < 0 ? 0.0 : (a[5] > 0 ? a[5] : 0.1));
     case DVS_PILE:
       return pile(seed, a[0] > 0 ? (int)a[0] : 1000, a[1] > 0 ? a[1] : 0.03,
                   a[2] > 0 ? a[2] : 0.06, a[3] > 0 ? a[3] : 0.3, a[4] > 0 ? (int)a[4] : 300);

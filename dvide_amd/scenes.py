@@ -111,9 +111,18 @@ def lattice_box(seed: int = 1001, per_side: int = 10) -> Scene:
     return generate(LATTICE_BOX, seed, per_side)
 
 
-def compaction(seed: int = 2002, n: int = 112000, width: float = 5.0) -> Scene:
-    """Settled granular pile under a lid driven down at 0.1 m/s (~250k contacts at 112k)."""
-    return generate(COMPACTION, seed, n, width)
+LID_FREE, LID_BILATERAL = 0, 1
+
+
+def compaction(seed: int = 2002, n: int = 112000, width: float = 5.0, lid: int = LID_FREE,
+               lid_mass: float = 0.0, lid_speed: float = 0.0) -> Scene:
+    """Settled granular pile under a rigid lid moving down at 0.1 m/s (~250k contacts at 112k).
+
+    LID_FREE: a free lid as heavy as the spheres, released at -0.1 m/s, warm
+    starting on (the reference's compaction driver); LID_BILATERAL: the lid's
+    DOFs driven by three bilateral rows (y at -0.1 m/s).
+    """
+    return generate(COMPACTION, seed, n, width, 0, lid, lid_mass, lid_speed)
 
 
 def pile(seed: int, target_contacts: int, rmin=0.03, rmax=0.06, mu=0.3) -> Scene:

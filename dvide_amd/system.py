@@ -49,6 +49,16 @@ class DvStats(ctypes.Structure):
     ]
 
 
+def gpu_visible() -> bool:
+    """True when the HIP runtime reports at least one device."""
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        return False
+    n = ctypes.c_int(0)
+    return hip.hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
+
+
 class DvideError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"dvide error {code}: {msg}")

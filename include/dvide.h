@@ -102,7 +102,9 @@ typedef struct dv_stats {
 
 /* --- scene lifetime ------------------------------------------------------ */
 
-/* System::System(int solverType), System.cu:62-133 */
+/* System::System(int solverType), System.cu:62-133.  Fails with DV_ERR_HIP
+ * when no HIP device is visible (there is no CPU fallback); dv_last_error(NULL)
+ * then names the cause. */
 int dv_scene_create(dv_scene** out, int solver);
 int dv_scene_destroy(dv_scene* s);
 const char* dv_last_error(const dv_scene* s);

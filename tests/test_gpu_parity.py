@@ -76,7 +76,7 @@ def test_assembly_bit_exact_lattice(gpu):
 
 
 def test_schur_product_bit_exact(gpu):
-    sc = scenes.compaction(2002, n=4000, width=1.5)
+    sc = scenes.compaction(2002, n=4000, width=1.5, lid=scenes.LID_BILATERAL)
     o, g = both(sc)
     nc = o.assemble()
     assert g.assemble() == nc and nc > 1000
@@ -107,7 +107,7 @@ def test_plain_trajectory_bit_exact(gpu):
                                                    (system.SOLVER_MIXED, 1e-4, 1e-3)])
 def test_fused_solvers_match_oracle(gpu, solver, tol, tol_gamma):
     """Graph-driven solvers against the oracle's FP64 APGD on the same step."""
-    sc = scenes.compaction(2002, n=4000, width=1.5)
+    sc = scenes.compaction(2002, n=4000, width=1.5, lid=scenes.LID_BILATERAL)
     o, g = both(sc, solver=solver, tol=tol, max_iterations=20000)
     o.step()
     g.step()
