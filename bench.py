@@ -85,6 +85,20 @@ def run_gpu(scene, solver, steps, warmup, profile_kernel=0):
         g.close()
 
 
+def copy_calibration(nbytes=1 << 30):
+    """A plain copy kernel of known size (torch elementwise copy, nbytes read
+    and nbytes written), run under rocprofv3 --pmc to scale FETCH_SIZE and
+    WRITE_SIZE into HBM bytes."""
+    import torch
+
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.add(x, 0.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
+
+
 def cpu_baseline(scene, steps):
     """The oracle (FP64 C++/OpenMP restatement) on the host cores."""
     import oracle
@@ -111,7 +125,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-mixed", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a 1 GiB copy kernel (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the contact time step is not sharded across GPUs "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)

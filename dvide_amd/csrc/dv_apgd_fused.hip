@@ -2,18 +2,22 @@
 // driven by a captured hipGraph: no host round trip per reduction.
 //
 // One solver pass (one Schur product plus the vector algebra around it) is
-// four kernels:
-//   P1 (contacts): y and N y from the last two iterates, g = N y + r,
-//       gamma_new = Pi(y - t g), the dot products of the objective and
-//       restart tests, and J_c = F_c^T gamma_new (the world-frame impulse of
-//       contact c on body A);
+// three kernels:
+//   P1 (contacts): first the decision of the previous pass -- fixed-order
+//       reduction of its per-workgroup partials, the Lipschitz test
+//       (APGD.cu:236-264) and either a backtracking step or the acceptance
+//       logic (APGD.cu:266-332) -- made redundantly by every workgroup from
+//       the same inputs; then y and N y from the last two iterates,
+//       g = N y + r, gamma_new = Pi(y - t g), the dot products of the
+//       objective and restart tests, and J_c = F_c^T gamma_new (the
+//       world-frame impulse of contact c on body A);
 //   P2 (bodies):   tmp_b = M_b^-1 (sum_A J_c - sum_B J_c + bilateral rows);
 //       bodies with many contacts are reduced by a whole workgroup;
 //   P3 (contacts): (N gamma_new)_c = F_c (tmp_A - tmp_B), gamma_new . N gamma_new
-//       and the residual's infinity norm (getResidual, APGD.cu:90-102);
-//   D  (1 block):  fixed-order reduction of the per-workgroup partials, the
-//       Lipschitz test (APGD.cu:236-264) and either a backtracking step or
-//       the acceptance logic (APGD.cu:266-332) applied to the state.
+//       and the residual's infinity norm (getResidual, APGD.cu:90-102).
+// State and partials are double-buffered by pass parity so that the
+// redundant decisions never race with their publication.  A one-wavefront
+// kernel decides the last pass of each chunk.
 // Kernel boundaries are the grid-wide barriers: on MI355X a kernel boundary
 // inside a graph costs ~1.6 us, less than a grid barrier of a persistent
 // cooperative kernel (3.7 us at 128 workgroups, ~12 us at 512).  The host
@@ -56,7 +60,8 @@ struct ApgdState {
   double L, t, theta, beta, best, objective;
   double sum[2 * kSlots];
   int it, bt, backtracks, restart;
-  int cur, prev, nxt, pad;
+  int cur, prev, nxt;
+  int pending;  // partials of a pass are waiting for their decision
   int hatSrc, done, converged, passes;
 };
 
@@ -72,7 +77,8 @@ constexpr int kTC = 1024;  // contact kernels: 16 wavefronts per workgroup
 constexpr int kTB = 256;   // body kernel
 constexpr int kTD = 64;    // decide kernels: one wavefront
 constexpr int kBacktrackCap = 1100;  // L doubles past 1e308 long before this
-constexpr int kPassesPerChunk = 16;
+constexpr int kPassesPerChunk = 16;  // even: a chunk's final state lands in st[0]
+static_assert(kPassesPerChunk % 2 == 0, "pass parity must return to 0 at a chunk end");
 constexpr int kMaxGridC = 256;  // reduceAll handles up to 4*64 partials
 
 template <typename TM>
@@ -106,8 +112,8 @@ struct Args {
   V4<TM>* tmp;
   double* gHat;         // gammaHat before any iterate is accepted (ones)
   TM* delta;            // mixed solver: gamma_new - y
-  double* part1;        // kSlots per workgroup
-  double* part3;
+  double* part1[2];     // kSlots per workgroup, by pass parity
+  double* part3[2];
 };
 
 // row offsets: contact c component k -> k*cap + c; bilateral b -> 3*cap + b
@@ -289,16 +295,17 @@ __global__ __launch_bounds__(kTC) void kSetup1(Args<TM> a) {
       storeJ(a, (int)q, F, (TM)x[0], (TM)x[1], (TM)x[2]);
     }
   }
-  writePartials<kTC>(acc, a.part1);
+  writePartials<kTC>(acc, a.part1[0]);
 }
 
 // P2: tmp = M^-1 D^T x for non-fixed bodies; X holds the bilateral rows of x.
-// which: 0 = ring[1] (d), 1 = ring[0] (gamma_0), 2 = ring[nxt] (gamma_new), 3 = gHat
+// which: 0 = ring[1] (d), 1 = ring[0] (gamma_0), 2 = ring[nxt] (gamma_new of
+// the pass whose state is st[sIdx]), 3 = gHat
 template <typename TM>
-__global__ __launch_bounds__(kTB) void kBody(Args<TM> a, int which) {
-  if (which == 2 && a.st->done) return;
+__global__ __launch_bounds__(kTB) void kBody(Args<TM> a, int which, int sIdx) {
+  if (which == 2 && a.st[sIdx].done) return;
   const SolveDims D = *a.dims;
-  const double* X = which == 0 ? a.ring[1] : which == 1 ? a.ring[0] : which == 2 ? a.ring[a.st->nxt] : a.gHat;
+  const double* X = which == 0 ? a.ring[1] : which == 1 ? a.ring[0] : which == 2 ? a.ring[a.st[sIdx].nxt] : a.gHat;
   const long bil0 = 3L * a.cap;
   GRID_LOOP(g, D.nb, kTB) {
     const TM mi = a.minv[g];
@@ -393,7 +400,7 @@ __global__ __launch_bounds__(kTC) void kSetup3(Args<TM> a) {
       storeJ(a, c, F, (TM)a.ring[0][ROW(a, 0, c)], (TM)a.ring[0][ROW(a, 1, c)], (TM)a.ring[0][ROW(a, 2, c)]);
     }
   }
-  writePartials<kTC>(acc, a.part3);
+  writePartials<kTC>(acc, a.part3[0]);
 }
 
 // N gamma_0 into rn[0]
@@ -419,7 +426,7 @@ __global__ __launch_bounds__(kTC) void kSetup5(Args<TM> a) {
 template <typename TM>
 __global__ __launch_bounds__(kTD) void kSetupDecide(Args<TM> a, int ng) {
   double tot[2 * kSlots];
-  reduceAll(a.part1, a.part3, ng, tot);
+  reduceAll(a.part1[0], a.part3[0], ng, tot);
   if (threadIdx.x == 0) {
     ApgdState S;
     double L = sqrt(tot[0]);
@@ -432,23 +439,108 @@ __global__ __launch_bounds__(kTD) void kSetupDecide(Args<TM> a, int ng) {
     S.objective = 0;
     for (int k = 0; k < 2 * kSlots; ++k) S.sum[k] = 0;
     S.it = 0; S.bt = 0; S.backtracks = 0; S.restart = 1;
-    S.cur = 0; S.prev = 0; S.nxt = 1; S.pad = 0; S.hatSrc = -1;  // -1: gHat (ones)
+    S.cur = 0; S.prev = 0; S.nxt = 1; S.pending = 0; S.hatSrc = -1;  // -1: gHat (ones)
     S.done = a.dims->maxIt <= 0 ? 1 : 0;
     S.converged = 0;
     S.passes = 0;
-    *a.st = S;
+    a.st[0] = S;
   }
 }
 
 // ---------------------------------------------------------------------------
-// one pass
+// the decision of a pass: Lipschitz test, backtracking or acceptance
+// (APGD.cu:236-332), from the pass's fixed-order partial sums tot[0..15]
+
+template <bool kMixed>
+__device__ void decideStep(ApgdState& S, const double* tot, const SolveDims& D) {
+  for (int k = 0; k < 2 * kSlots; ++k) S.sum[k] = tot[k];
+  const double* s1 = tot;           // P1 slots
+  const double* s3 = tot + kSlots;  // P3 slots
+  S.passes++;
+  const double L = S.L;
+  bool backtrack;
+  if (kMixed) {
+    backtrack = (s3[6] - s1[1]) >= L * s1[4];
+  } else {
+    const double obj1 = 0.5 * s3[6] + s1[0];
+    const double nrm = sqrt(s1[4]);
+    double obj2 = 0.5 * s1[1] + s1[2];
+    obj2 += s1[3] + 0.5 * L * (nrm * nrm);
+    backtrack = obj1 >= obj2;
+  }
+  if (backtrack && S.bt < kBacktrackCap) {
+    S.bt++;
+    S.backtracks++;
+    S.L = 2.0 * L;
+    S.t = 1.0 / S.L;
+    return;
+  }
+  const double theta = S.theta;
+  double thetaNew = (-(theta * theta) + theta * sqrt(theta * theta + 4.0)) / 2.0;
+  S.beta = theta * (1.0 - theta) / (theta * theta + thetaNew);
+  const double res = s3[7];
+  if (res < S.best) {  // gammaHat := gamma_new, kept in its ring slot
+    S.best = res;
+    S.hatSrc = S.nxt;
+  }
+  if (S.best < D.tol) {
+    S.done = 1;
+    S.converged = 1;
+    return;
+  }
+  if (s1[5] > 0) {
+    S.restart = 1;
+    thetaNew = 1.0;
+  } else {
+    S.restart = 0;
+  }
+  S.L = 0.9 * L;
+  S.t = 1.0 / S.L;
+  S.theta = thetaNew;
+  S.prev = S.cur;
+  S.cur = S.nxt;
+  // next slot: the one of the four not holding gamma_k, gamma_{k-1} or gammaHat
+  for (int k = 0; k < 4; ++k)
+    if (k != S.cur && k != S.prev && k != S.hatSrc) { S.nxt = k; break; }
+  S.it++;
+  S.bt = 0;
+  if (S.it >= D.maxIt) S.done = 1;
+}
+
+// ---------------------------------------------------------------------------
+// one pass: P1 (parity par) reads the state st[par] and, if a pass is
+// pending, the partials part1/part3[par ^ 1] of that pass; wavefront 0 of
+// every workgroup applies the decision to its copy (same inputs, same fixed
+// reduction order, same result), block 0 publishes the new state in
+// st[par ^ 1] for P2, P3 and the next pass, and P1 writes part1[par].
 
 template <typename TM>
-__global__ __launch_bounds__(kTC) void kP1(Args<TM> a) {
+__device__ __forceinline__ ApgdState passState(const Args<TM>& a, int par, const SolveDims& D) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
-  const ApgdState S = *a.st;
-  if (S.done) return;
+  __shared__ ApgdState sh;
+  if (threadIdx.x < 64) {
+    ApgdState S = a.st[par];
+    if (S.pending && !S.done) {
+      double tot[2 * kSlots];
+      reduceAll(a.part1[par ^ 1], a.part3[par ^ 1], gridDim.x, tot);
+      decideStep<kMixed>(S, tot, D);
+    }
+    S.pending = S.done ? 0 : 1;
+    if (threadIdx.x == 0) {
+      sh = S;
+      if (blockIdx.x == 0) a.st[par ^ 1] = S;
+    }
+  }
+  __syncthreads();
+  return sh;
+}
+
+template <typename TM>
+__global__ __launch_bounds__(kTC) void kP1(Args<TM> a, int par) {
+  constexpr bool kMixed = sizeof(TM) < sizeof(double);
   const SolveDims D = *a.dims;
+  const ApgdState S = passState(a, par, D);
+  if (S.done) return;
   const long items = (long)D.nc + D.nbil;
   const double* Gk = a.ring[S.cur];
   const double* Gkm1 = a.ring[S.prev];
@@ -509,13 +601,13 @@ __global__ __launch_bounds__(kTC) void kP1(Args<TM> a) {
       storeJ(a, (int)q, F, (TM)gn[0], (TM)gn[1], (TM)gn[2]);
     }
   }
-  writePartials<kTC>(acc, a.part1);
+  writePartials<kTC>(acc, a.part1[par]);
 }
 
 template <typename TM>
-__global__ __launch_bounds__(kTC) void kP3(Args<TM> a) {
+__global__ __launch_bounds__(kTC) void kP3(Args<TM> a, int par) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
-  const ApgdState S = *a.st;
+  const ApgdState S = a.st[par ^ 1];
   if (S.done) return;
   const SolveDims D = *a.dims;
   const long items = (long)D.nc + D.nbil;
@@ -560,74 +652,25 @@ __global__ __launch_bounds__(kTC) void kP3(Args<TM> a) {
       acc[7] = fmax(acc[7], residualContact(g0, g1, g2, w0, w1, w2, D.mu, D.gdiff));
     }
   }
-  writePartials<kTC>(acc, a.part3);
+  writePartials<kTC>(acc, a.part3[par]);
 }
 
-// Lipschitz test, backtracking or acceptance (APGD.cu:236-332)
+// Chunk-end decision: the state of the last pass of a chunk sits in
+// st[par ^ 1] and its partials in part1/part3[par]; decide it, so that the
+// host sees the final state and the next chunk's first P1 has nothing pending.
 template <typename TM>
-__global__ __launch_bounds__(kTD) void kDecide(Args<TM> a, int ng) {
+__global__ __launch_bounds__(kTD) void kDecide(Args<TM> a, int par, int ng) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
-  if (a.st->done) return;
-  double tot[2 * kSlots];
-  reduceAll(a.part1, a.part3, ng, tot);
-  if (threadIdx.x != 0) return;
-  ApgdState S = *a.st;
-  const SolveDims D = *a.dims;
-  for (int k = 0; k < 2 * kSlots; ++k) S.sum[k] = tot[k];
-  const double* s1 = tot;           // P1 slots
-  const double* s3 = tot + kSlots;  // P3 slots
-  S.passes++;
-  const double L = S.L;
-  bool backtrack;
-  if (kMixed) {
-    backtrack = (s3[6] - s1[1]) >= L * s1[4];
-  } else {
-    const double obj1 = 0.5 * s3[6] + s1[0];
-    const double nrm = sqrt(s1[4]);
-    double obj2 = 0.5 * s1[1] + s1[2];
-    obj2 += s1[3] + 0.5 * L * (nrm * nrm);
-    backtrack = obj1 >= obj2;
+  ApgdState S = a.st[par ^ 1];
+  if (!S.pending) return;
+  if (!S.done) {
+    double tot[2 * kSlots];
+    reduceAll(a.part1[par], a.part3[par], ng, tot);
+    const SolveDims D = *a.dims;
+    decideStep<kMixed>(S, tot, D);
   }
-  if (backtrack && S.bt < kBacktrackCap) {
-    S.bt++;
-    S.backtracks++;
-    S.L = 2.0 * L;
-    S.t = 1.0 / S.L;
-    *a.st = S;
-    return;
-  }
-  const double theta = S.theta;
-  double thetaNew = (-(theta * theta) + theta * sqrt(theta * theta + 4.0)) / 2.0;
-  S.beta = theta * (1.0 - theta) / (theta * theta + thetaNew);
-  const double res = s3[7];
-  if (res < S.best) {  // gammaHat := gamma_new, kept in its ring slot
-    S.best = res;
-    S.hatSrc = S.nxt;
-  }
-  if (S.best < D.tol) {
-    S.done = 1;
-    S.converged = 1;
-    *a.st = S;
-    return;
-  }
-  if (s1[5] > 0) {
-    S.restart = 1;
-    thetaNew = 1.0;
-  } else {
-    S.restart = 0;
-  }
-  S.L = 0.9 * L;
-  S.t = 1.0 / S.L;
-  S.theta = thetaNew;
-  S.prev = S.cur;
-  S.cur = S.nxt;
-  // next slot: the one of the four not holding gamma_k, gamma_{k-1} or gammaHat
-  for (int k = 0; k < 4; ++k)
-    if (k != S.cur && k != S.prev && k != S.hatSrc) { S.nxt = k; break; }
-  S.it++;
-  S.bt = 0;
-  if (S.it >= D.maxIt) S.done = 1;
-  *a.st = S;
+  S.pending = 0;
+  if (threadIdx.x == 0) a.st[par ^ 1] = S;
 }
 
 // ---------------------------------------------------------------------------
@@ -636,7 +679,7 @@ __global__ __launch_bounds__(kTD) void kDecide(Args<TM> a, int ng) {
 template <typename TM>
 __global__ __launch_bounds__(kTC) void kFin1(Args<TM> a) {
   const SolveDims D = *a.dims;
-  const ApgdState S = *a.st;
+  const ApgdState S = a.st[0];
   const long items = (long)D.nc + D.nbil;
   const double* hatV = S.hatSrc >= 0 ? a.ring[S.hatSrc] : a.gHat;
   double acc[kSlots];
@@ -659,7 +702,7 @@ __global__ __launch_bounds__(kTC) void kFin1(Args<TM> a) {
       storeJ(a, (int)q, F, (TM)x[0], (TM)x[1], (TM)x[2]);
     }
   }
-  writePartials<kTC>(acc, a.part1);
+  writePartials<kTC>(acc, a.part1[0]);
 }
 
 template <typename TM>
@@ -681,14 +724,14 @@ __global__ __launch_bounds__(kTC) void kFin3(Args<TM> a) {
                 a.gHat[ROW(a, 2, c)] * (double)n2;
     }
   }
-  writePartials<kTC>(acc, a.part3);
+  writePartials<kTC>(acc, a.part3[0]);
 }
 
 template <typename TM>
 __global__ __launch_bounds__(kTD) void kFinDecide(Args<TM> a, int ng) {
   double tot[2 * kSlots];
-  reduceAll(a.part1, a.part3, ng, tot);
-  if (threadIdx.x == 0) a.st->objective = 0.5 * tot[kSlots + 1] + tot[0];
+  reduceAll(a.part1[0], a.part3[0], ng, tot);
+  if (threadIdx.x == 0) a.st[0].objective = 0.5 * tot[kSlots + 1] + tot[0];
 }
 
 __global__ __launch_bounds__(256) void toFloat(const double* __restrict__ x, float* __restrict__ y, long n) {
@@ -705,21 +748,25 @@ struct Plan {
   int gc, gb;
 };
 
-// One chunk of n passes.  With `ev` (stream launches only: events recorded
-// inside a captured graph cannot be timed), events bracket kernel `prof` of
-// pass k (ev[2k], ev[2k+1]).
+// One chunk of n (even) passes, pass k with parity k & 1, and the decision
+// of the last pass.  With `ev` (stream launches only: events recorded inside
+// a captured graph cannot be timed), events bracket kernel `prof` of pass k
+// (ev[2k], ev[2k+1]); prof 4 times the chunk-end decision (ev[0], ev[1]).
 template <typename TM>
 void enqueuePasses(const Plan<TM>& P, hipStream_t st, int n, hipEvent_t* ev, int prof) {
   for (int k = 0; k < n; ++k) {
-    for (int kern = 1; kern <= 4; ++kern) {
+    const int par = k & 1;
+    for (int kern = 1; kern <= 3; ++kern) {
       if (ev && prof == kern) DV_HIP(hipEventRecord(ev[2 * k], st));
-      if (kern == 1) hipLaunchKernelGGL(kP1<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a);
-      if (kern == 2) hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.a, 2);
-      if (kern == 3) hipLaunchKernelGGL(kP3<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a);
-      if (kern == 4) hipLaunchKernelGGL(kDecide<TM>, dim3(1), dim3(kTD), 0, st, P.a, P.gc);
+      if (kern == 1) hipLaunchKernelGGL(kP1<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a, par);
+      if (kern == 2) hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.a, 2, par ^ 1);
+      if (kern == 3) hipLaunchKernelGGL(kP3<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a, par);
       if (ev && prof == kern) DV_HIP(hipEventRecord(ev[2 * k + 1], st));
     }
   }
+  if (ev && prof == 4) DV_HIP(hipEventRecord(ev[0], st));
+  hipLaunchKernelGGL(kDecide<TM>, dim3(1), dim3(kTD), 0, st, P.a, (n - 1) & 1, P.gc);
+  if (ev && prof == 4) DV_HIP(hipEventRecord(ev[1], st));
 }
 
 template <typename TM>
@@ -727,9 +774,9 @@ void runSolve(Scene& s, Plan<TM>& P, GraphCache& cache) {
   const hipStream_t st = s.stream;
   // setup (APGD.cu:195-223)
   hipLaunchKernelGGL(kSetup1<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a);
-  hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.a, 0);
+  hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.a, 0, 0);
   hipLaunchKernelGGL(kSetup3<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a);
-  hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.a, 1);
+  hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.a, 1, 0);
   hipLaunchKernelGGL(kSetup5<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a);
   hipLaunchKernelGGL(kSetupDecide<TM>, dim3(1), dim3(kTD), 0, st, P.a, P.gc);
   // iterations: graph chunks until the state reports done
@@ -767,7 +814,8 @@ void runSolve(Scene& s, Plan<TM>& P, GraphCache& cache) {
     DV_HIP(hipStreamSynchronize(st));
     if (timed) {
       // passes of this chunk that did work are its first (passes - before)
-      const int ran = std::min(s.stateHost->passes - passesBefore, kPassesPerChunk);
+      int ran = std::min(s.stateHost->passes - passesBefore, kPassesPerChunk);
+      if (prof == 4) ran = std::min(ran, 1);  // one chunk-end decision per chunk
       for (int k = 0; k < ran; ++k) {
         float ms = 0;
         DV_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
@@ -781,7 +829,7 @@ void runSolve(Scene& s, Plan<TM>& P, GraphCache& cache) {
   }
   // gamma := gammaHat and the objective
   hipLaunchKernelGGL(kFin1<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a);
-  hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.a, 3);
+  hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.a, 3, 0);
   hipLaunchKernelGGL(kFin3<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a);
   hipLaunchKernelGGL(kFinDecide<TM>, dim3(1), dim3(kTD), 0, st, P.a, P.gc);
   DV_HIP(hipMemcpyAsync(s.stateHost, P.a.st, sizeof(ApgdState), hipMemcpyDeviceToHost, st));
@@ -823,8 +871,10 @@ void setupPlan(Scene& s, Plan<TM>& P, TM* WM, long RsCap) {
   uintptr_t p = (reinterpret_cast<uintptr_t>(after) + 31) & ~uintptr_t(31);
   a.J = reinterpret_cast<V4<TM>*>(p);
   a.tmp = a.J + std::max<long>(s.ncCap, 1);
-  a.part1 = s.partials.p;
-  a.part3 = s.partials.p + (long)kSlots * kMaxGridC;
+  for (int k = 0; k < 2; ++k) {
+    a.part1[k] = s.partials.p + (long)(2 * k) * kSlots * kMaxGridC;
+    a.part3[k] = s.partials.p + (long)(2 * k + 1) * kSlots * kMaxGridC;
+  }
   const long itemsCap = (long)s.ncCap + s.nbil;
   // one 16-wavefront workgroup per CU (grid-stride beyond): equal work per CU
   P.gc = std::max(1, std::min({blocksFor(itemsCap, kTC), s.numCU, kMaxGridC}));
@@ -839,12 +889,12 @@ size_t workElems(Scene& s, long RsCap) {
 
 void solveFused(Scene& s, bool mixed) {
   if (!s.state) {
-    DV_HIP(hipMalloc(&s.state, sizeof(ApgdState)));
+    DV_HIP(hipMalloc(&s.state, 2 * sizeof(ApgdState)));  // by pass parity
     DV_HIP(hipMalloc(&s.solveDims, sizeof(SolveDims)));
     DV_HIP(hipHostMalloc(&s.stateHost, sizeof(ApgdState), hipHostMallocDefault));
     DV_HIP(hipHostMalloc(&s.dimsHost, sizeof(SolveDims), hipHostMallocDefault));
   }
-  s.partials.reserve((size_t)2 * kSlots * kMaxGridC);
+  s.partials.reserve((size_t)4 * kSlots * kMaxGridC);
   const double cnt = (double)((long)s.nc + s.nbil);
   SolveDims& D = *s.dimsHost;
   D.nc = s.nc; D.nbil = s.nbil; D.nb = s.nb; D.maxIt = s.maxIt;
