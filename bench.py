@@ -99,6 +99,130 @@ def copy_calibration(nbytes=1 << 30):
     print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
 
 
+def _gen_many(specs):
+    """Scenes generated on the host cores in parallel (the generator releases
+    the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from dvide_amd import scenes
+
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(lambda a: scenes.pile(*a), specs))
+
+
+def batched(n_scenes=512, steps=2, warmup=1, seed=4004):
+    """512 independent piles, contacts log-uniform in [1k, 100k], radii
+    U[0.03, 0.06] m, mu U[0.1, 0.5]; one mixed-precision APGD solve of all of
+    them per step in the same graph launches."""
+    from dvide_amd import system
+
+    rng = np.random.default_rng(seed)
+    targets = np.exp(rng.uniform(np.log(1e3), np.log(1e5), n_scenes)).astype(int)
+    mus = rng.uniform(0.1, 0.5, n_scenes)
+    t0 = time.perf_counter()
+    scs = _gen_many([(seed + i, int(targets[i]), 0.03, 0.06, float(mus[i])) for i in range(n_scenes)])
+    gen = time.perf_counter() - t0
+    systems = [system.from_scene(sc, system.SOLVER_MIXED, tol=TOL) for sc in scs]
+    batch = system.Batch(system.SOLVER_MIXED)
+    for g in systems:
+        batch.add(g)
+    for _ in range(warmup):
+        batch.step()
+    hip = _hip()
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(steps):
+        batch.step()
+        stats.append(batch.stats())
+    hip.hipDeviceSynchronize()
+    dt = time.perf_counter() - t0
+    contacts = np.mean([s.total_contacts for s in stats])
+    out = {
+        "scenes": n_scenes,
+        "scene_steps_per_s": round(n_scenes * steps / dt, 2),
+        "contact_steps_per_s": round(contacts * steps / dt, 1),
+        "ms_per_batch_step": round(1e3 * dt / steps, 2),
+        "contacts_per_step": int(contacts),
+        "max_iterations": int(max(s.max_iterations for s in stats)),
+        "mean_iterations": round(float(np.mean([s.total_iterations / n_scenes for s in stats])), 1),
+        "ms_assembly": round(float(np.mean([s.ms_assembly for s in stats])), 2),
+        "ms_solve": round(float(np.mean([s.ms_solve for s in stats])), 2),
+        "scene_generation_s": round(gen, 1),
+    }
+    batch.close()
+    for g in systems:
+        g.close()
+    return out
+
+
+def served(n_scenes=256, n_requests=512, load=0.7, seed=5005, max_slots=64):
+    """Poisson stream of time-step requests over 256 resident scenes, sizes
+    log-uniform in [1k, 200k] contacts, radii U[0.02, 0.03] m, mu 0.4; the
+    server admits by free slot and HBM budget and overlaps the solves.  The
+    rate is `load` times the throughput measured in a closed-loop warm-up."""
+    from dvide_amd import system
+
+    rng = np.random.default_rng(seed)
+    targets = np.exp(rng.uniform(np.log(1e3), np.log(2e5), n_scenes)).astype(int)
+    scs = _gen_many([(seed + i, int(targets[i]), 0.02, 0.03, 0.4) for i in range(n_scenes)])
+    systems = [system.from_scene(sc, system.SOLVER_MIXED, tol=TOL) for sc in scs]
+    srv = system.Server(system.SOLVER_MIXED, max_slots=max_slots)
+    # closed loop: every scene once, as fast as the server completes them
+    t0 = time.perf_counter()
+    for g in systems:
+        srv.submit(g)
+    srv.drain()
+    capacity = n_scenes / (time.perf_counter() - t0)
+    rate = load * capacity
+    # open loop: Poisson arrivals at `rate`, scenes drawn uniformly; a request
+    # for a scene with one in flight waits for it
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    which = rng.integers(0, n_scenes, n_requests)
+    busy, queued, ticket_of = set(), {}, {}
+    submitted_at, latency = {}, []
+    start = time.perf_counter()
+    nxt = 0
+    while len(latency) < n_requests:
+        now = time.perf_counter() - start
+        while nxt < n_requests and arrivals[nxt] <= now:
+            queued.setdefault(int(which[nxt]), []).append(arrivals[nxt])
+            nxt += 1
+        for sc_i in list(queued):
+            if sc_i not in busy and queued[sc_i]:
+                t_arr = queued[sc_i].pop(0)
+                tk = srv.submit(systems[sc_i])
+                ticket_of[tk] = sc_i
+                submitted_at[tk] = t_arr
+                busy.add(sc_i)
+            if not queued[sc_i]:
+                del queued[sc_i]
+        if srv.pending():
+            for tk in srv.poll():
+                latency.append(time.perf_counter() - start - submitted_at.pop(tk))
+                busy.discard(ticket_of.pop(tk))
+        elif nxt < n_requests:
+            time.sleep(max(0.0, min(0.001, arrivals[nxt] - now)))
+    total = time.perf_counter() - start
+    lat = np.array(latency) * 1e3
+    srv.close()
+    for g in systems:
+        g.close()
+    return {
+        "scenes": n_scenes,
+        "requests": n_requests,
+        "offered_rate_per_s": round(rate, 2),
+        "throughput_per_s": round(n_requests / total, 2),
+        "closed_loop_capacity_per_s": round(capacity, 2),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                       "p99": round(float(np.percentile(lat, 99)), 2),
+                       "mean": round(float(lat.mean()), 2)},
+        "max_slots": max_slots,
+        "contacts_mean": int(np.mean(targets)),
+    }
+
+
 def cpu_baseline(scene, steps):
     """The oracle (FP64 C++/OpenMP restatement) on the host cores."""
     import oracle
@@ -125,6 +249,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-mixed", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-scene batched solve")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served request stream")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB copy kernel (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -210,6 +336,10 @@ def main():
         line["mixed_vs_plain_fp64"] = round(value / line["plain_fp64"]["value"], 2)
     if "fp64" in line:
         line["mixed_vs_fp64"] = round(value / line["fp64"]["value"], 2)
+    if not args.no_batch and not args.only_mixed:
+        line["batched"] = batched()
+    if not args.no_serve and not args.only_mixed:
+        line["served"] = served()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(scene, args.cpu_steps)
     print(json.dumps(line), flush=True)

@@ -1,28 +1,35 @@
 // APGD (APGD::solve, APGD.cu:164-361) as a device-resident state machine
-// driven by a captured hipGraph: no host round trip per reduction.
+// driven by a captured hipGraph: no host round trip per reduction.  One
+// engine solves one scene or a batch of independent scenes in the same
+// launches.
 //
 // One solver pass (one Schur product plus the vector algebra around it) is
-// three kernels:
-//   P1 (contacts): first the decision of the previous pass -- fixed-order
-//       reduction of its per-workgroup partials, the Lipschitz test
-//       (APGD.cu:236-264) and either a backtracking step or the acceptance
-//       logic (APGD.cu:266-332) -- made redundantly by every workgroup from
-//       the same inputs; then y and N y from the last two iterates,
-//       g = N y + r, gamma_new = Pi(y - t g), the dot products of the
-//       objective and restart tests, and J_c = F_c^T gamma_new (the
-//       world-frame impulse of contact c on body A);
+// three kernels over all active scenes:
+//   P1 (contacts): y and N y from the last two iterates, g = N y + r,
+//       gamma_new = Pi(y - t g), the dot products of the objective and
+//       restart tests, and J_c = F_c^T gamma_new (the world-frame impulse of
+//       contact c on body A);
 //   P2 (bodies):   tmp_b = M_b^-1 (sum_A J_c - sum_B J_c + bilateral rows);
 //       bodies with many contacts are reduced by a whole workgroup;
 //   P3 (contacts): (N gamma_new)_c = F_c (tmp_A - tmp_B), gamma_new . N gamma_new
-//       and the residual's infinity norm (getResidual, APGD.cu:90-102).
-// State and partials are double-buffered by pass parity so that the
-// redundant decisions never race with their publication.  A one-wavefront
-// kernel decides the last pass of each chunk.
-// Kernel boundaries are the grid-wide barriers: on MI355X a kernel boundary
-// inside a graph costs ~1.6 us, less than a grid barrier of a persistent
-// cooperative kernel (3.7 us at 128 workgroups, ~12 us at 512).  The host
-// launches the graph of a chunk of passes, reads one flag, and repeats; passes
-// after convergence exit immediately.
+//       and the residual's infinity norm (getResidual, APGD.cu:90-102);
+// followed by each scene's decision: a fixed-order reduction of its partial
+// sums, the Lipschitz test (APGD.cu:236-264) and either a backtracking step
+// or the acceptance logic (APGD.cu:266-332).  For a single scene the
+// decision is folded into the next P1: every workgroup reduces the same
+// partials in the same order, reaches the same state, and workgroup 0
+// publishes it; a one-wavefront kernel decides the last pass of a chunk.  A
+// batch runs one decision kernel per pass with a wavefront per scene.
+//
+// Work is cut into tiles of one scene (kTC rows or kTB bodies); a scene's
+// partial sums are kept per tile and reduced in tile order, so a scene gives
+// bit-identical results alone and in any batch.  State and partials are
+// double-buffered by pass parity, so no decision races with its
+// publication.  Kernel boundaries are the grid-wide barriers: on MI355X a
+// kernel boundary inside a graph costs ~1.6 us, less than a grid barrier of
+// a persistent cooperative kernel (3.7 us at 128 workgroups, ~12 us at 512).
+// The host launches the graph of a chunk of passes, reads the scenes'
+// states, and repeats; scenes that have converged skip their tiles.
 //
 // The same arithmetic as the reference, less traffic:
 //   * N y is not recomputed: with y_{k+1} = (1+beta) g_{k+1} - beta g_k,
@@ -61,7 +68,7 @@ struct ApgdState {
   double sum[2 * kSlots];
   int it, bt, backtracks, restart;
   int cur, prev, nxt;
-  int pending;  // partials of a pass are waiting for their decision
+  int pending;  // partials of a pass are waiting for their decision (folded mode)
   int hatSrc, done, converged, passes;
 };
 
@@ -73,24 +80,27 @@ struct SolveDims {
 
 namespace {
 
-constexpr int kTC = 1024;  // contact kernels: 16 wavefronts per workgroup
-constexpr int kTB = 256;   // body kernel
-constexpr int kTD = 64;    // decide kernels: one wavefront
+constexpr int kTC = 1024;  // contact kernels: 16 wavefronts, one tile of rows
+constexpr int kTB = 256;   // body kernel: one tile of bodies
+constexpr int kTD = 64;    // decide kernels: one wavefront per scene
 constexpr int kBacktrackCap = 1100;  // L doubles past 1e308 long before this
-constexpr int kPassesPerChunk = 16;  // even: a chunk's final state lands in st[0]
+constexpr int kPassesPerChunk = 16;  // even: a chunk ends on parity 0
 static_assert(kPassesPerChunk % 2 == 0, "pass parity must return to 0 at a chunk end");
-constexpr int kMaxGridC = 256;  // reduceAll handles up to 4*64 partials
 
 template <typename TM>
 struct V4 {
   TM x, y, z, w;
 };
 
+// Per-scene view of the solver, one entry of the device-resident scene table.
 template <typename TM>
 struct Args {
-  const SolveDims* dims;
-  ApgdState* st;
-  const TM* frame;      // SoA stride cap: n (FP64) or n, u (FP32)
+  SolveDims dims;
+  int nTiles;           // row tiles of this solve (partial sums per slot)
+  int active;           // the slot holds a scene being solved
+  ApgdState* st;        // [2], by pass parity
+  const double* frame64;  // assembled frames: 9 SoA arrays n, u, v of stride cap
+  TM* frame;            // frames the passes read: frame64's n (FP64) or an FP32 n, u copy
   long cap;             // contact capacity: row-vector stride
   const int* cA;
   const int* cB;
@@ -102,7 +112,8 @@ struct Args {
   const int* bilB;
   const int* heavy;
   const int* counters;  // [2] = number of heavy bodies
-  const TM* minv;
+  const double* minv64;
+  TM* minv;             // minv64 (FP64) or its FP32 copy
   const double* rFlat;  // assembled r, reference layout
   TM* r;                // r, component-major
   double* gammaIO;      // flat gamma: previous values in, result out
@@ -112,8 +123,19 @@ struct Args {
   V4<TM>* tmp;
   double* gHat;         // gammaHat before any iterate is accepted (ones)
   TM* delta;            // mixed solver: gamma_new - y
-  double* part1[2];     // kSlots per workgroup, by pass parity
+  double* part1[2];     // kSlots per tile, by pass parity
   double* part3[2];
+};
+
+// Launch-wide tables (device memory, so that a captured graph stays valid
+// while their contents change).
+struct Tables {
+  const Tile* items;    // row tiles (kTC rows of one slot)
+  const Tile* bodies;   // body tiles (kTB bodies of one slot)
+  const int2* heavy;    // (slot, body) of bodies reduced by a workgroup
+  const int* counts;    // [0] row tiles, [1] body tiles, [2] heavy bodies
+  const int* slots;     // the set's slots (one decide workgroup each)
+  ApgdState* out;       // per slot: the state after the latest decision
 };
 
 // row offsets: contact c component k -> k*cap + c; bilateral b -> 3*cap + b
@@ -158,36 +180,42 @@ template <int NT>
 __device__ void writePartials(double (&acc)[kSlots], double* part) {
   __shared__ double tot[kSlots];
   blockReduce<NT, kSlots>(acc, tot);
-  if (threadIdx.x < kSlots) part[(long)blockIdx.x * kSlots + threadIdx.x] = tot[threadIdx.x];
+  if (threadIdx.x < kSlots) part[threadIdx.x] = tot[threadIdx.x];
 }
 
-// fixed-order totals of the partials of `ng` <= 4*64 workgroups from P1 and
-// P3, by one wavefront: out[0..7] from part1, out[8..15] from part3 (valid
-// in every lane).  All loads are issued before any is consumed.
+// Fixed-order totals of the per-tile partials of one scene by one wavefront
+// (all 64 lanes): out[0..7] from part1, out[8..15] from part3, valid in every
+// lane.  Lane l adds tiles l, l+64, l+128, ... in order; the lanes are then
+// combined by a butterfly.  Loads are issued four tiles at a time.
 __device__ void reduceAll(const double* part1, const double* part3, int ng, double* out) {
-  const int lane = threadIdx.x;
-  double x[4][2 * kSlots];
+  const int lane = threadIdx.x & 63;
+  double v[2 * kSlots];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int w = lane + 64 * k;
-    const int ws = w < ng ? w : 0;
-#pragma unroll
-    for (int s = 0; s < kSlots; ++s) {
-      x[k][s] = part1[(long)ws * kSlots + s];
-      x[k][kSlots + s] = part3[(long)ws * kSlots + s];
-    }
-  }
-#pragma unroll
-  for (int s = 0; s < 2 * kSlots; ++s) {
-    const bool isMax = (s % kSlots == kSlots - 1);
-    double v = 0.0;
+  for (int s = 0; s < 2 * kSlots; ++s) v[s] = 0.0;
+  for (int base = 0; base < ng; base += 256) {
+    double x[4][2 * kSlots];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const double xv = (lane + 64 * k) < ng ? x[k][s] : 0.0;
-      v = isMax ? fmax(v, xv) : v + xv;
+      const int w = base + lane + 64 * k;
+      const int ws = w < ng ? w : 0;
+#pragma unroll
+      for (int s = 0; s < kSlots; ++s) {
+        x[k][s] = part1[(long)ws * kSlots + s];
+        x[k][kSlots + s] = part3[(long)ws * kSlots + s];
+      }
     }
-    out[s] = isMax ? waveMax(v) : waveSum(v);
+#pragma unroll
+    for (int s = 0; s < 2 * kSlots; ++s) {
+      const bool isMax = (s % kSlots == kSlots - 1);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double xv = (base + lane + 64 * k) < ng ? x[k][s] : 0.0;
+        v[s] = isMax ? fmax(v[s], xv) : v[s] + xv;
+      }
+    }
   }
+#pragma unroll
+  for (int s = 0; s < 2 * kSlots; ++s) out[s] = (s % kSlots == kSlots - 1) ? waveMax(v[s]) : waveSum(v[s]);
 }
 
 // ---------------------------------------------------------------------------
@@ -259,61 +287,95 @@ __device__ __forceinline__ void zero(double (&acc)[kSlots]) {
   for (int s = 0; s < kSlots; ++s) acc[s] = 0.0;
 }
 
-#define GRID_LOOP(q, n, NT) for (long q = (long)blockIdx.x * NT + threadIdx.x; q < (n); q += (long)gridDim.x * NT)
+// loop of a workgroup over the row tiles (uniform across the workgroup)
+#define TILE_LOOP(T) for (int ti_ = blockIdx.x, nt_ = (T).counts[0]; ti_ < nt_; ti_ += gridDim.x)
+
+// partial-sum slots of a tile
+#define TILE_PART(p, tile) ((p) + (long)((tile).start / kTC) * kSlots)
 
 // ---------------------------------------------------------------------------
 // setup (APGD.cu:195-223)
 
 // r to component-major; gamma_0, gammaHat_0 = 1, d = gamma_0 - gammaHat_0 (in
-// ring[1]); J = F^T d; |d|^2
+// ring[1]); J = F^T d; |d|^2.  The mixed solver first rounds the frames.
 template <typename TM>
-__global__ __launch_bounds__(kTC) void kSetup1(Args<TM> a) {
-  const SolveDims D = *a.dims;
-  const long items = (long)D.nc + D.nbil, n3 = 3L * D.nc;
-  double acc[kSlots];
-  zero(acc);
-  GRID_LOOP(q, items, kTC) {
-    const bool bil = q >= D.nc;
-    const int nr = bil ? 1 : 3;
-    double x[3] = {0, 0, 0};
-    for (int rr = 0; rr < nr; ++rr) {
-      const long f = bil ? (q - D.nc) : (D.nbil + 3L * q + rr);  // flat (reference) row
-      const long o = bil ? BROW(a, q - D.nc) : ROW(a, rr, q);
-      a.r[o] = (TM)a.rFlat[f];
-      // flat entries 0..3nc-1 start as (1,0,0) triples, without the
-      // bilateral offset (APGD.cu:116-122); the rest keep their old values
-      const double gv = (!D.warm && f < n3) ? ((f % 3 == 0) ? 1.0 : 0.0) : a.gammaIO[f];
-      a.ring[0][o] = gv;
-      a.gHat[o] = 1.0;
-      x[rr] = 1.0 * gv + (-1.0) * 1.0;
-      a.ring[1][o] = x[rr];
-      acc[0] += x[rr] * x[rr];
+__global__ __launch_bounds__(kTC) void kSetup1(const Args<TM>* __restrict__ A, Tables T) {
+  constexpr bool kMixed = sizeof(TM) < sizeof(double);
+  TILE_LOOP(T) {
+    const Tile tile = T.items[ti_];
+    const Args<TM>& a = A[tile.scene];
+    const SolveDims& D = a.dims;
+    const long items = (long)D.nc + D.nbil, n3 = 3L * D.nc;
+    const long q = (long)tile.start + threadIdx.x;
+    double acc[kSlots];
+    zero(acc);
+    if (q < items) {
+      const bool bil = q >= D.nc;
+      const int nr = bil ? 1 : 3;
+      double x[3] = {0, 0, 0};
+      for (int rr = 0; rr < nr; ++rr) {
+        const long f = bil ? (q - D.nc) : (D.nbil + 3L * q + rr);  // flat (reference) row
+        const long o = bil ? BROW(a, q - D.nc) : ROW(a, rr, q);
+        a.r[o] = (TM)a.rFlat[f];
+        // flat entries 0..3nc-1 start as (1,0,0) triples, without the
+        // bilateral offset (APGD.cu:116-122); the rest keep their old values
+        const double gv = (!D.warm && f < n3) ? ((f % 3 == 0) ? 1.0 : 0.0) : a.gammaIO[f];
+        a.ring[0][o] = gv;
+        a.gHat[o] = 1.0;
+        x[rr] = 1.0 * gv + (-1.0) * 1.0;
+        a.ring[1][o] = x[rr];
+        acc[0] += x[rr] * x[rr];
+      }
+      if (!bil) {
+        if (kMixed) {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) a.frame[k * a.cap + q] = (TM)a.frame64[k * a.cap + q];
+        }
+        TM F[9];
+        ldFrame(a, (int)q, D.geomMode, F);
+        storeJ(a, (int)q, F, (TM)x[0], (TM)x[1], (TM)x[2]);
+      }
     }
-    if (!bil) {
-      TM F[9];
-      ldFrame(a, (int)q, D.geomMode, F);
-      storeJ(a, (int)q, F, (TM)x[0], (TM)x[1], (TM)x[2]);
-    }
+    writePartials<kTC>(acc, TILE_PART(a.part1[0], tile));
   }
-  writePartials<kTC>(acc, a.part1[0]);
 }
 
 // P2: tmp = M^-1 D^T x for non-fixed bodies; X holds the bilateral rows of x.
-// which: 0 = ring[1] (d), 1 = ring[0] (gamma_0), 2 = ring[nxt] (gamma_new of
-// the pass whose state is st[sIdx]), 3 = gHat
+// which: 0 = ring[1] (d; also rounds M^-1 for the mixed solver), 1 = ring[0]
+// (gamma_0), 2 = ring[nxt] (gamma_new of the pass whose state is st[sIdx]),
+// 3 = gHat
 template <typename TM>
-__global__ __launch_bounds__(kTB) void kBody(Args<TM> a, int which, int sIdx) {
-  if (which == 2 && a.st[sIdx].done) return;
-  const SolveDims D = *a.dims;
-  const double* X = which == 0 ? a.ring[1] : which == 1 ? a.ring[0] : which == 2 ? a.ring[a.st[sIdx].nxt] : a.gHat;
-  const long bil0 = 3L * a.cap;
-  GRID_LOOP(g, D.nb, kTB) {
-    const TM mi = a.minv[g];
+__device__ __forceinline__ const double* bodySource(const Args<TM>& a, int which, int sIdx) {
+  return which == 0 ? a.ring[1] : which == 1 ? a.ring[0] : which == 2 ? a.ring[a.st[sIdx].nxt] : a.gHat;
+}
+
+template <typename TM>
+__device__ __forceinline__ TM bodyMinv(const Args<TM>& a, int g, int which) {
+  constexpr bool kMixed = sizeof(TM) < sizeof(double);
+  if (kMixed && which == 0) {
+    const TM mi = (TM)a.minv64[g];
+    a.minv[g] = mi;
+    return mi;
+  }
+  return a.minv[g];
+}
+
+template <typename TM>
+__global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tables T, int which, int sIdx) {
+  for (int ti = blockIdx.x, nt = T.counts[1]; ti < nt; ti += gridDim.x) {
+    const Tile tile = T.bodies[ti];
+    const Args<TM>& a = A[tile.scene];
+    if (which == 2 && a.st[sIdx].done) continue;
+    const int g = tile.start + threadIdx.x;
+    if (g >= a.dims.nb) continue;
+    const double* X = bodySource(a, which, sIdx);
+    const long bil0 = 3L * a.cap;
     const int k0 = a.iStart[g], k1 = a.iStart[g + 1];
+    if (k1 - k0 > kHeavyDegree) continue;  // reduced below by a workgroup
+    const TM mi = bodyMinv(a, g, which);
     V4<TM> out;
     out.x = 0; out.y = 0; out.z = 0; out.w = 0;
     if (mi != (TM)0) {
-      if (k1 - k0 > kHeavyDegree) continue;  // reduced below by a workgroup
       TM s0 = 0, s1 = 0, s2 = 0;
       for (int k = a.bilIncStart[g]; k < a.bilIncStart[g + 1]; ++k) {
         const int e = a.bilInc[k];
@@ -344,11 +406,22 @@ __global__ __launch_bounds__(kTB) void kBody(Args<TM> a, int which, int sIdx) {
     }
     a.tmp[g] = out;
   }
-  const int nHeavy = a.counters[2];
-  for (int hq = blockIdx.x; hq < nHeavy; hq += gridDim.x) {
-    const int g = a.heavy[hq];
-    const TM mi = a.minv[g];
-    if (mi == (TM)0) continue;
+  for (int hq = blockIdx.x, nh = T.counts[2]; hq < nh; hq += gridDim.x) {
+    const int2 ref = T.heavy[hq];
+    const Args<TM>& a = A[ref.x];
+    if (which == 2 && a.st[sIdx].done) continue;
+    const int g = ref.y;
+    const TM mi = bodyMinv(a, g, which);
+    if (mi == (TM)0) {
+      if (threadIdx.x == 0) {
+        V4<TM> out;
+        out.x = 0; out.y = 0; out.z = 0; out.w = 0;
+        a.tmp[g] = out;
+      }
+      continue;
+    }
+    const double* X = bodySource(a, which, sIdx);
+    const long bil0 = 3L * a.cap;
     const int k0 = a.iStart[g], k1 = a.iStart[g + 1];
     double acc[kSlots];
     zero(acc);
@@ -381,34 +454,43 @@ __global__ __launch_bounds__(kTB) void kBody(Args<TM> a, int which, int sIdx) {
 
 // |N d|^2 into part3[1]; J = F^T gamma_0
 template <typename TM>
-__global__ __launch_bounds__(kTC) void kSetup3(Args<TM> a) {
-  const SolveDims D = *a.dims;
-  const long items = (long)D.nc + D.nbil;
-  double acc[kSlots];
-  zero(acc);
-  GRID_LOOP(q, items, kTC) {
-    if (q >= D.nc) {
-      const double v = (double)bilRow(a, (int)(q - D.nc));
-      acc[1] += v * v;
-    } else {
-      const int c = (int)q;
-      TM F[9];
-      ldFrame(a, c, D.geomMode, F);
-      TM n0, n1, n2;
-      contactRows(a, c, F, n0, n1, n2);
-      acc[1] += (double)n0 * (double)n0 + (double)n1 * (double)n1 + (double)n2 * (double)n2;
-      storeJ(a, c, F, (TM)a.ring[0][ROW(a, 0, c)], (TM)a.ring[0][ROW(a, 1, c)], (TM)a.ring[0][ROW(a, 2, c)]);
+__global__ __launch_bounds__(kTC) void kSetup3(const Args<TM>* __restrict__ A, Tables T) {
+  TILE_LOOP(T) {
+    const Tile tile = T.items[ti_];
+    const Args<TM>& a = A[tile.scene];
+    const SolveDims& D = a.dims;
+    const long items = (long)D.nc + D.nbil;
+    const long q = (long)tile.start + threadIdx.x;
+    double acc[kSlots];
+    zero(acc);
+    if (q < items) {
+      if (q >= D.nc) {
+        const double v = (double)bilRow(a, (int)(q - D.nc));
+        acc[1] += v * v;
+      } else {
+        const int c = (int)q;
+        TM F[9];
+        ldFrame(a, c, D.geomMode, F);
+        TM n0, n1, n2;
+        contactRows(a, c, F, n0, n1, n2);
+        acc[1] += (double)n0 * (double)n0 + (double)n1 * (double)n1 + (double)n2 * (double)n2;
+        storeJ(a, c, F, (TM)a.ring[0][ROW(a, 0, c)], (TM)a.ring[0][ROW(a, 1, c)], (TM)a.ring[0][ROW(a, 2, c)]);
+      }
     }
+    writePartials<kTC>(acc, TILE_PART(a.part3[0], tile));
   }
-  writePartials<kTC>(acc, a.part3[0]);
 }
 
 // N gamma_0 into rn[0]
 template <typename TM>
-__global__ __launch_bounds__(kTC) void kSetup5(Args<TM> a) {
-  const SolveDims D = *a.dims;
-  const long items = (long)D.nc + D.nbil;
-  GRID_LOOP(q, items, kTC) {
+__global__ __launch_bounds__(kTC) void kSetup5(const Args<TM>* __restrict__ A, Tables T) {
+  TILE_LOOP(T) {
+    const Tile tile = T.items[ti_];
+    const Args<TM>& a = A[tile.scene];
+    const SolveDims& D = a.dims;
+    const long items = (long)D.nc + D.nbil;
+    const long q = (long)tile.start + threadIdx.x;
+    if (q >= items) continue;
     if (q >= D.nc) {
       a.rn[0][BROW(a, q - D.nc)] = bilRow(a, (int)(q - D.nc));
     } else {
@@ -422,11 +504,14 @@ __global__ __launch_bounds__(kTC) void kSetup5(Args<TM> a) {
   }
 }
 
-// L_0 = |N d| / |d|, t_0 = 1/L_0 and the initial state (APGD.cu:208-223)
+// L_0 = |N d| / |d|, t_0 = 1/L_0 and the initial state (APGD.cu:208-223),
+// written to both parity slots; one wavefront per scene
 template <typename TM>
-__global__ __launch_bounds__(kTD) void kSetupDecide(Args<TM> a, int ng) {
+__global__ __launch_bounds__(kTD) void kSetupDecide(const Args<TM>* __restrict__ A, Tables T) {
+  const int slot = T.slots[blockIdx.x];
+  const Args<TM>& a = A[slot];
   double tot[2 * kSlots];
-  reduceAll(a.part1[0], a.part3[0], ng, tot);
+  reduceAll(a.part1[0], a.part3[0], a.nTiles, tot);
   if (threadIdx.x == 0) {
     ApgdState S;
     double L = sqrt(tot[0]);
@@ -440,10 +525,12 @@ __global__ __launch_bounds__(kTD) void kSetupDecide(Args<TM> a, int ng) {
     for (int k = 0; k < 2 * kSlots; ++k) S.sum[k] = 0;
     S.it = 0; S.bt = 0; S.backtracks = 0; S.restart = 1;
     S.cur = 0; S.prev = 0; S.nxt = 1; S.pending = 0; S.hatSrc = -1;  // -1: gHat (ones)
-    S.done = a.dims->maxIt <= 0 ? 1 : 0;
+    S.done = a.dims.maxIt <= 0 ? 1 : 0;
     S.converged = 0;
     S.passes = 0;
     a.st[0] = S;
+    a.st[1] = S;
+    T.out[slot] = S;
   }
 }
 
@@ -507,23 +594,19 @@ __device__ void decideStep(ApgdState& S, const double* tot, const SolveDims& D) 
   if (S.it >= D.maxIt) S.done = 1;
 }
 
-// ---------------------------------------------------------------------------
-// one pass: P1 (parity par) reads the state st[par] and, if a pass is
-// pending, the partials part1/part3[par ^ 1] of that pass; wavefront 0 of
-// every workgroup applies the decision to its copy (same inputs, same fixed
-// reduction order, same result), block 0 publishes the new state in
-// st[par ^ 1] for P2, P3 and the next pass, and P1 writes part1[par].
-
+// Folded decision (single scene): P1 of parity par reads st[par] and, if a
+// pass is pending, the partials part1/part3[par ^ 1] of that pass; wavefront
+// 0 of every workgroup applies the decision to its copy, workgroup 0
+// publishes the result in st[par ^ 1] for P2, P3 and the next pass.
 template <typename TM>
-__device__ __forceinline__ ApgdState passState(const Args<TM>& a, int par, const SolveDims& D) {
+__device__ void passState(const Args<TM>& a, int par, ApgdState& sh) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
-  __shared__ ApgdState sh;
   if (threadIdx.x < 64) {
     ApgdState S = a.st[par];
     if (S.pending && !S.done) {
       double tot[2 * kSlots];
-      reduceAll(a.part1[par ^ 1], a.part3[par ^ 1], gridDim.x, tot);
-      decideStep<kMixed>(S, tot, D);
+      reduceAll(a.part1[par ^ 1], a.part3[par ^ 1], a.nTiles, tot);
+      decideStep<kMixed>(S, tot, a.dims);
     }
     S.pending = S.done ? 0 : 1;
     if (threadIdx.x == 0) {
@@ -532,211 +615,274 @@ __device__ __forceinline__ ApgdState passState(const Args<TM>& a, int par, const
     }
   }
   __syncthreads();
-  return sh;
-}
-
-template <typename TM>
-__global__ __launch_bounds__(kTC) void kP1(Args<TM> a, int par) {
-  constexpr bool kMixed = sizeof(TM) < sizeof(double);
-  const SolveDims D = *a.dims;
-  const ApgdState S = passState(a, par, D);
-  if (S.done) return;
-  const long items = (long)D.nc + D.nbil;
-  const double* Gk = a.ring[S.cur];
-  const double* Gkm1 = a.ring[S.prev];
-  const TM* Nk = a.rn[S.cur];
-  const TM* Nkm1 = a.rn[S.prev];
-  double* Gn = a.ring[S.nxt];
-  const double bp = 1.0 + S.beta, bm = -S.beta;
-  const double t = S.t;
-  double acc[kSlots];
-  zero(acc);
-  GRID_LOOP(q, items, kTC) {
-    const bool bil = q >= D.nc;
-    const int nr = bil ? 1 : 3;
-    double y[3] = {0, 0, 0}, ny[3] = {0, 0, 0}, gk[3] = {0, 0, 0}, gg[3] = {0, 0, 0}, gn[3] = {0, 0, 0},
-           rv[3] = {0, 0, 0};
-    long o[3] = {0, 0, 0};
-#pragma unroll
-    for (int rr = 0; rr < 3; ++rr) {
-      if (rr < nr) {
-        o[rr] = bil ? BROW(a, q - D.nc) : ROW(a, rr, q);
-        const long f = o[rr];
-        gk[rr] = Gk[f];
-        rv[rr] = (double)a.r[f];
-        if (S.restart) {
-          y[rr] = gk[rr];
-          ny[rr] = (double)Nk[f];
-        } else {
-          y[rr] = bp * gk[rr] + bm * Gkm1[f];
-          ny[rr] = bp * (double)Nk[f] + bm * (double)Nkm1[f];
-        }
-        gg[rr] = 1.0 * ny[rr] + 1.0 * rv[rr];
-        gn[rr] = 1.0 * y[rr] + (-t) * gg[rr];
-      }
-    }
-    if (!bil) projectCone<double>(gn[0], gn[1], gn[2], D.mu);
-#pragma unroll
-    for (int rr = 0; rr < 3; ++rr) {
-      if (rr < nr) {
-        const long f = o[rr];
-        Gn[f] = gn[rr];
-        const double dy = gn[rr] - y[rr];
-        if (kMixed) {
-          a.delta[f] = (TM)dy;
-          acc[1] += dy * ny[rr];
-        } else {
-          acc[0] += gn[rr] * rv[rr];
-          acc[1] += y[rr] * ny[rr];
-          acc[2] += y[rr] * rv[rr];
-          acc[3] += gg[rr] * dy;
-        }
-        acc[4] += dy * dy;
-        acc[5] += gg[rr] * (gn[rr] - gk[rr]);
-      }
-    }
-    if (!bil) {
-      TM F[9];
-      ldFrame(a, (int)q, D.geomMode, F);
-      storeJ(a, (int)q, F, (TM)gn[0], (TM)gn[1], (TM)gn[2]);
-    }
-  }
-  writePartials<kTC>(acc, a.part1[par]);
-}
-
-template <typename TM>
-__global__ __launch_bounds__(kTC) void kP3(Args<TM> a, int par) {
-  constexpr bool kMixed = sizeof(TM) < sizeof(double);
-  const ApgdState S = a.st[par ^ 1];
-  if (S.done) return;
-  const SolveDims D = *a.dims;
-  const long items = (long)D.nc + D.nbil;
-  const double* Gn = a.ring[S.nxt];
-  TM* Nn = a.rn[S.nxt];
-  double acc[kSlots];
-  zero(acc);
-  GRID_LOOP(q, items, kTC) {
-    if (q >= D.nc) {
-      const long f = BROW(a, q - D.nc);
-      const TM nv = bilRow(a, (int)(q - D.nc));
-      Nn[f] = nv;
-      const double gv = Gn[f];
-      acc[6] += (kMixed ? (double)a.delta[f] : gv) * (double)nv;
-      const double w = (double)nv + (double)a.r[f];
-      double e;
-      if (D.nc) {
-        const double z = 1.0 * gv + (-D.gdiff) * w;
-        const double ig = 1.0 / D.gdiff;
-        e = fabs(ig * gv + (-ig) * z);
-      } else {
-        e = fabs(w);
-      }
-      acc[7] = fmax(acc[7], e);
-    } else {
-      const int c = (int)q;
-      TM F[9];
-      ldFrame(a, c, D.geomMode, F);
-      TM n0, n1, n2;
-      contactRows(a, c, F, n0, n1, n2);
-      const long f0 = ROW(a, 0, c), f1 = ROW(a, 1, c), f2 = ROW(a, 2, c);
-      Nn[f0] = n0; Nn[f1] = n1; Nn[f2] = n2;
-      const double g0 = Gn[f0], g1 = Gn[f1], g2 = Gn[f2];
-      if (kMixed)
-        acc[6] += (double)a.delta[f0] * (double)n0 + (double)a.delta[f1] * (double)n1 +
-                  (double)a.delta[f2] * (double)n2;
-      else
-        acc[6] += g0 * (double)n0 + g1 * (double)n1 + g2 * (double)n2;
-      const double w0 = (double)n0 + (double)a.r[f0];
-      const double w1 = (double)n1 + (double)a.r[f1];
-      const double w2 = (double)n2 + (double)a.r[f2];
-      acc[7] = fmax(acc[7], residualContact(g0, g1, g2, w0, w1, w2, D.mu, D.gdiff));
-    }
-  }
-  writePartials<kTC>(acc, a.part3[par]);
-}
-
-// Chunk-end decision: the state of the last pass of a chunk sits in
-// st[par ^ 1] and its partials in part1/part3[par]; decide it, so that the
-// host sees the final state and the next chunk's first P1 has nothing pending.
-template <typename TM>
-__global__ __launch_bounds__(kTD) void kDecide(Args<TM> a, int par, int ng) {
-  constexpr bool kMixed = sizeof(TM) < sizeof(double);
-  ApgdState S = a.st[par ^ 1];
-  if (!S.pending) return;
-  if (!S.done) {
-    double tot[2 * kSlots];
-    reduceAll(a.part1[par], a.part3[par], ng, tot);
-    const SolveDims D = *a.dims;
-    decideStep<kMixed>(S, tot, D);
-  }
-  S.pending = 0;
-  if (threadIdx.x == 0) a.st[par ^ 1] = S;
 }
 
 // ---------------------------------------------------------------------------
-// gamma := gammaHat (APGD.cu:358) and objectiveCCP (APGD.cu:353-354)
+// one pass (parity par): the state of the pass is st[par ^ 1]
 
 template <typename TM>
-__global__ __launch_bounds__(kTC) void kFin1(Args<TM> a) {
-  const SolveDims D = *a.dims;
-  const ApgdState S = a.st[0];
-  const long items = (long)D.nc + D.nbil;
-  const double* hatV = S.hatSrc >= 0 ? a.ring[S.hatSrc] : a.gHat;
-  double acc[kSlots];
-  zero(acc);
-  GRID_LOOP(q, items, kTC) {
-    const bool bil = q >= D.nc;
-    const int nr = bil ? 1 : 3;
-    double x[3] = {0, 0, 0};
-    for (int rr = 0; rr < nr; ++rr) {
-      const long o = bil ? BROW(a, q - D.nc) : ROW(a, rr, q);
-      const long f = bil ? (q - D.nc) : (D.nbil + 3L * q + rr);
-      x[rr] = hatV[o];
-      a.gHat[o] = x[rr];
-      a.gammaIO[f] = x[rr];
-      acc[0] += x[rr] * (double)a.r[o];
+__global__ __launch_bounds__(kTC) void kP1(const Args<TM>* __restrict__ A, Tables T, int par, int fold) {
+  constexpr bool kMixed = sizeof(TM) < sizeof(double);
+  __shared__ ApgdState shS;
+  if (fold) passState(A[0], par, shS);
+  TILE_LOOP(T) {
+    const Tile tile = T.items[ti_];
+    const Args<TM>& a = A[tile.scene];
+    const ApgdState S = fold ? shS : a.st[par ^ 1];
+    if (S.done) continue;
+    const SolveDims& D = a.dims;
+    const long items = (long)D.nc + D.nbil;
+    const long q = (long)tile.start + threadIdx.x;
+    double acc[kSlots];
+    zero(acc);
+    if (q < items) {
+      const double* Gk = a.ring[S.cur];
+      const double* Gkm1 = a.ring[S.prev];
+      const TM* Nk = a.rn[S.cur];
+      const TM* Nkm1 = a.rn[S.prev];
+      double* Gn = a.ring[S.nxt];
+      const double bp = 1.0 + S.beta, bm = -S.beta;
+      const double t = S.t;
+      const bool bil = q >= D.nc;
+      const int nr = bil ? 1 : 3;
+      double y[3] = {0, 0, 0}, ny[3] = {0, 0, 0}, gk[3] = {0, 0, 0}, gg[3] = {0, 0, 0}, gn[3] = {0, 0, 0},
+             rv[3] = {0, 0, 0};
+      long o[3] = {0, 0, 0};
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr) {
+        if (rr < nr) {
+          o[rr] = bil ? BROW(a, q - D.nc) : ROW(a, rr, q);
+          const long f = o[rr];
+          gk[rr] = Gk[f];
+          rv[rr] = (double)a.r[f];
+          if (S.restart) {
+            y[rr] = gk[rr];
+            ny[rr] = (double)Nk[f];
+          } else {
+            y[rr] = bp * gk[rr] + bm * Gkm1[f];
+            ny[rr] = bp * (double)Nk[f] + bm * (double)Nkm1[f];
+          }
+          gg[rr] = 1.0 * ny[rr] + 1.0 * rv[rr];
+          gn[rr] = 1.0 * y[rr] + (-t) * gg[rr];
+        }
+      }
+      if (!bil) projectCone<double>(gn[0], gn[1], gn[2], D.mu);
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr) {
+        if (rr < nr) {
+          const long f = o[rr];
+          Gn[f] = gn[rr];
+          const double dy = gn[rr] - y[rr];
+          if (kMixed) {
+            a.delta[f] = (TM)dy;
+            acc[1] += dy * ny[rr];
+          } else {
+            acc[0] += gn[rr] * rv[rr];
+            acc[1] += y[rr] * ny[rr];
+            acc[2] += y[rr] * rv[rr];
+            acc[3] += gg[rr] * dy;
+          }
+          acc[4] += dy * dy;
+          acc[5] += gg[rr] * (gn[rr] - gk[rr]);
+        }
+      }
+      if (!bil) {
+        TM F[9];
+        ldFrame(a, (int)q, D.geomMode, F);
+        storeJ(a, (int)q, F, (TM)gn[0], (TM)gn[1], (TM)gn[2]);
+      }
     }
-    if (!bil) {
-      TM F[9];
-      ldFrame(a, (int)q, D.geomMode, F);
-      storeJ(a, (int)q, F, (TM)x[0], (TM)x[1], (TM)x[2]);
-    }
+    writePartials<kTC>(acc, TILE_PART(a.part1[par], tile));
   }
-  writePartials<kTC>(acc, a.part1[0]);
 }
 
 template <typename TM>
-__global__ __launch_bounds__(kTC) void kFin3(Args<TM> a) {
-  const SolveDims D = *a.dims;
-  const long items = (long)D.nc + D.nbil;
-  double acc[kSlots];
-  zero(acc);
-  GRID_LOOP(q, items, kTC) {
-    if (q >= D.nc) {
-      acc[1] += a.gHat[BROW(a, q - D.nc)] * (double)bilRow(a, (int)(q - D.nc));
-    } else {
-      const int c = (int)q;
-      TM F[9];
-      ldFrame(a, c, D.geomMode, F);
-      TM n0, n1, n2;
-      contactRows(a, c, F, n0, n1, n2);
-      acc[1] += a.gHat[ROW(a, 0, c)] * (double)n0 + a.gHat[ROW(a, 1, c)] * (double)n1 +
-                a.gHat[ROW(a, 2, c)] * (double)n2;
+__global__ __launch_bounds__(kTC) void kP3(const Args<TM>* __restrict__ A, Tables T, int par) {
+  constexpr bool kMixed = sizeof(TM) < sizeof(double);
+  TILE_LOOP(T) {
+    const Tile tile = T.items[ti_];
+    const Args<TM>& a = A[tile.scene];
+    const ApgdState& S = a.st[par ^ 1];
+    if (S.done) continue;
+    const SolveDims& D = a.dims;
+    const long items = (long)D.nc + D.nbil;
+    const long q = (long)tile.start + threadIdx.x;
+    double acc[kSlots];
+    zero(acc);
+    if (q < items) {
+      const double* Gn = a.ring[S.nxt];
+      TM* Nn = a.rn[S.nxt];
+      if (q >= D.nc) {
+        const long f = BROW(a, q - D.nc);
+        const TM nv = bilRow(a, (int)(q - D.nc));
+        Nn[f] = nv;
+        const double gv = Gn[f];
+        acc[6] += (kMixed ? (double)a.delta[f] : gv) * (double)nv;
+        const double w = (double)nv + (double)a.r[f];
+        double e;
+        if (D.nc) {
+          const double z = 1.0 * gv + (-D.gdiff) * w;
+          const double ig = 1.0 / D.gdiff;
+          e = fabs(ig * gv + (-ig) * z);
+        } else {
+          e = fabs(w);
+        }
+        acc[7] = fmax(acc[7], e);
+      } else {
+        const int c = (int)q;
+        TM F[9];
+        ldFrame(a, c, D.geomMode, F);
+        TM n0, n1, n2;
+        contactRows(a, c, F, n0, n1, n2);
+        const long f0 = ROW(a, 0, c), f1 = ROW(a, 1, c), f2 = ROW(a, 2, c);
+        Nn[f0] = n0; Nn[f1] = n1; Nn[f2] = n2;
+        const double g0 = Gn[f0], g1 = Gn[f1], g2 = Gn[f2];
+        if (kMixed)
+          acc[6] += (double)a.delta[f0] * (double)n0 + (double)a.delta[f1] * (double)n1 +
+                    (double)a.delta[f2] * (double)n2;
+        else
+          acc[6] += g0 * (double)n0 + g1 * (double)n1 + g2 * (double)n2;
+        const double w0 = (double)n0 + (double)a.r[f0];
+        const double w1 = (double)n1 + (double)a.r[f1];
+        const double w2 = (double)n2 + (double)a.r[f2];
+        acc[7] = fmax(acc[7], residualContact(g0, g1, g2, w0, w1, w2, D.mu, D.gdiff));
+      }
     }
+    writePartials<kTC>(acc, TILE_PART(a.part3[par], tile));
   }
-  writePartials<kTC>(acc, a.part3[0]);
+}
+
+// Decision of the pass of parity par for every scene (one wavefront each).
+// Batches (fold 0) decide every pass: state st[par ^ 1] -> st[par].  A
+// single scene (fold 1) decides here only the last pass of a chunk, in
+// place, so that the next chunk's first P1 has nothing pending.  Both
+// record the state in T.out for the host.
+template <typename TM>
+__global__ __launch_bounds__(kTD) void kDecide(const Args<TM>* __restrict__ A, Tables T, int par, int fold) {
+  constexpr bool kMixed = sizeof(TM) < sizeof(double);
+  const Args<TM>& a = A[blockIdx.x];  // one workgroup per slot
+  if (!a.active) return;
+  const int src = par ^ 1;
+  const int dst = fold ? src : par;
+  ApgdState S = a.st[src];
+  const bool run = fold ? (S.pending && !S.done) : !S.done;
+  if (run) {
+    double tot[2 * kSlots];
+    reduceAll(a.part1[par], a.part3[par], a.nTiles, tot);
+    decideStep<kMixed>(S, tot, a.dims);
+  }
+  S.pending = 0;
+  if (threadIdx.x == 0) {
+    a.st[dst] = S;
+    T.out[blockIdx.x] = S;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// gamma := gammaHat (APGD.cu:358) and objectiveCCP (APGD.cu:353-354); the
+// final state is st[fin]
+
+template <typename TM>
+__global__ __launch_bounds__(kTC) void kFin1(const Args<TM>* __restrict__ A, Tables T, int fin) {
+  TILE_LOOP(T) {
+    const Tile tile = T.items[ti_];
+    const Args<TM>& a = A[tile.scene];
+    const SolveDims& D = a.dims;
+    const long items = (long)D.nc + D.nbil;
+    const long q = (long)tile.start + threadIdx.x;
+    const int hatSrc = a.st[fin].hatSrc;
+    const double* hatV = hatSrc >= 0 ? a.ring[hatSrc] : a.gHat;
+    double acc[kSlots];
+    zero(acc);
+    if (q < items) {
+      const bool bil = q >= D.nc;
+      const int nr = bil ? 1 : 3;
+      double x[3] = {0, 0, 0};
+      for (int rr = 0; rr < nr; ++rr) {
+        const long o = bil ? BROW(a, q - D.nc) : ROW(a, rr, q);
+        const long f = bil ? (q - D.nc) : (D.nbil + 3L * q + rr);
+        x[rr] = hatV[o];
+        a.gHat[o] = x[rr];
+        a.gammaIO[f] = x[rr];
+        acc[0] += x[rr] * (double)a.r[o];
+      }
+      if (!bil) {
+        TM F[9];
+        ldFrame(a, (int)q, D.geomMode, F);
+        storeJ(a, (int)q, F, (TM)x[0], (TM)x[1], (TM)x[2]);
+      }
+    }
+    writePartials<kTC>(acc, TILE_PART(a.part1[0], tile));
+  }
 }
 
 template <typename TM>
-__global__ __launch_bounds__(kTD) void kFinDecide(Args<TM> a, int ng) {
+__global__ __launch_bounds__(kTC) void kFin3(const Args<TM>* __restrict__ A, Tables T) {
+  TILE_LOOP(T) {
+    const Tile tile = T.items[ti_];
+    const Args<TM>& a = A[tile.scene];
+    const SolveDims& D = a.dims;
+    const long items = (long)D.nc + D.nbil;
+    const long q = (long)tile.start + threadIdx.x;
+    double acc[kSlots];
+    zero(acc);
+    if (q < items) {
+      if (q >= D.nc) {
+        acc[1] += a.gHat[BROW(a, q - D.nc)] * (double)bilRow(a, (int)(q - D.nc));
+      } else {
+        const int c = (int)q;
+        TM F[9];
+        ldFrame(a, c, D.geomMode, F);
+        TM n0, n1, n2;
+        contactRows(a, c, F, n0, n1, n2);
+        acc[1] += a.gHat[ROW(a, 0, c)] * (double)n0 + a.gHat[ROW(a, 1, c)] * (double)n1 +
+                  a.gHat[ROW(a, 2, c)] * (double)n2;
+      }
+    }
+    writePartials<kTC>(acc, TILE_PART(a.part3[0], tile));
+  }
+}
+
+template <typename TM>
+__global__ __launch_bounds__(kTD) void kFinDecide(const Args<TM>* __restrict__ A, Tables T, int fin) {
+  const int slot = T.slots[blockIdx.x];
+  const Args<TM>& a = A[slot];
   double tot[2 * kSlots];
-  reduceAll(a.part1[0], a.part3[0], ng, tot);
-  if (threadIdx.x == 0) a.st[0].objective = 0.5 * tot[kSlots + 1] + tot[0];
+  reduceAll(a.part1[0], a.part3[0], a.nTiles, tot);
+  if (threadIdx.x == 0) {
+    ApgdState S = a.st[fin];
+    S.objective = 0.5 * tot[kSlots + 1] + tot[0];
+    a.st[fin] = S;
+    T.out[slot] = S;
+  }
 }
 
-__global__ __launch_bounds__(256) void toFloat(const double* __restrict__ x, float* __restrict__ y, long n) {
-  long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) y[i] = (float)x[i];
+// (slot, body) list of the heavy bodies of the slots of a set, in slot-list
+// order; one workgroup, slots in blocks of 1024 with a workgroup-wide prefix
+// sum
+template <typename TM>
+__global__ __launch_bounds__(1024) void kHeavyTable(const Args<TM>* __restrict__ A, const int* slots, int nSlots,
+                                                     int2* refs, int* counts) {
+  __shared__ int sc[1024];
+  int base = 0;
+  for (int s0 = 0; s0 < nSlots; s0 += 1024) {
+    const int i = s0 + (int)threadIdx.x;
+    const int s = i < nSlots ? slots[i] : 0;
+    const int n = i < nSlots ? A[s].counters[2] : 0;
+    sc[threadIdx.x] = n;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {  // inclusive scan
+      const int v = (int)threadIdx.x >= d ? sc[threadIdx.x - d] : 0;
+      __syncthreads();
+      sc[threadIdx.x] += v;
+      __syncthreads();
+    }
+    const int off = base + sc[threadIdx.x] - n;
+    for (int h = 0; h < n; ++h) refs[off + h] = make_int2(s, A[s].heavy[h]);
+    base += sc[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[2] = base;
 }
 
 // ---------------------------------------------------------------------------
@@ -744,96 +890,33 @@ __global__ __launch_bounds__(256) void toFloat(const double* __restrict__ x, flo
 
 template <typename TM>
 struct Plan {
-  Args<TM> a;
-  int gc, gb;
+  const Args<TM>* A;
+  Tables T;
+  int gc, gb, nSlots, fold;
 };
 
-// One chunk of n (even) passes, pass k with parity k & 1, and the decision
-// of the last pass.  With `ev` (stream launches only: events recorded inside
-// a captured graph cannot be timed), events bracket kernel `prof` of pass k
-// (ev[2k], ev[2k+1]); prof 4 times the chunk-end decision (ev[0], ev[1]).
+// One chunk of n (even) passes, pass k with parity k & 1.  A batch decides
+// after every pass; a single scene decides inside P1 and once more at the
+// chunk end.  With `ev` (stream launches only: events recorded inside a
+// captured graph cannot be timed), events bracket kernel `prof` of pass k
+// (ev[2k], ev[2k+1]); with the folded decision, prof 4 times the chunk-end
+// decision (ev[0], ev[1]).
 template <typename TM>
 void enqueuePasses(const Plan<TM>& P, hipStream_t st, int n, hipEvent_t* ev, int prof) {
   for (int k = 0; k < n; ++k) {
     const int par = k & 1;
-    for (int kern = 1; kern <= 3; ++kern) {
-      if (ev && prof == kern) DV_HIP(hipEventRecord(ev[2 * k], st));
-      if (kern == 1) hipLaunchKernelGGL(kP1<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a, par);
-      if (kern == 2) hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.a, 2, par ^ 1);
-      if (kern == 3) hipLaunchKernelGGL(kP3<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a, par);
-      if (ev && prof == kern) DV_HIP(hipEventRecord(ev[2 * k + 1], st));
+    for (int kern = 1; kern <= 4; ++kern) {
+      if (kern == 4 && P.fold && k != n - 1) continue;
+      const bool timed = ev && prof == kern;
+      const int e = (kern == 4 && P.fold) ? 0 : 2 * k;
+      if (timed) DV_HIP(hipEventRecord(ev[e], st));
+      if (kern == 1) hipLaunchKernelGGL(kP1<TM>, dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par, P.fold);
+      if (kern == 2) hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.A, P.T, 2, par ^ 1);
+      if (kern == 3) hipLaunchKernelGGL(kP3<TM>, dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
+      if (kern == 4) hipLaunchKernelGGL(kDecide<TM>, dim3(P.nSlots), dim3(kTD), 0, st, P.A, P.T, par, P.fold);
+      if (timed) DV_HIP(hipEventRecord(ev[e + 1], st));
     }
   }
-  if (ev && prof == 4) DV_HIP(hipEventRecord(ev[0], st));
-  hipLaunchKernelGGL(kDecide<TM>, dim3(1), dim3(kTD), 0, st, P.a, (n - 1) & 1, P.gc);
-  if (ev && prof == 4) DV_HIP(hipEventRecord(ev[1], st));
-}
-
-template <typename TM>
-void runSolve(Scene& s, Plan<TM>& P, GraphCache& cache) {
-  const hipStream_t st = s.stream;
-  // setup (APGD.cu:195-223)
-  hipLaunchKernelGGL(kSetup1<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a);
-  hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.a, 0, 0);
-  hipLaunchKernelGGL(kSetup3<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a);
-  hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.a, 1, 0);
-  hipLaunchKernelGGL(kSetup5<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a);
-  hipLaunchKernelGGL(kSetupDecide<TM>, dim3(1), dim3(kTD), 0, st, P.a, P.gc);
-  // iterations: graph chunks until the state reports done
-  hipEvent_t* ev = nullptr;
-  const int prof = s.profKernel;
-  if (prof) {
-    while (s.profEv.size() < 2 * (size_t)kPassesPerChunk) {
-      hipEvent_t e;
-      DV_HIP(hipEventCreate(&e));
-      s.profEv.push_back(e);
-    }
-    ev = s.profEv.data();
-  }
-  if (!s.noGraph && !cache.matches(&P, sizeof(P))) {
-    cache.reset();
-    hipGraph_t graph;
-    DV_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    enqueuePasses<TM>(P, st, kPassesPerChunk, nullptr, 0);
-    DV_HIP(hipStreamEndCapture(st, &graph));
-    hipError_t e = hipGraphInstantiate(&cache.exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) throw HipError{e, "hipGraphInstantiate(apgd passes)"};
-    cache.setKey(&P, sizeof(P));
-  }
-  const long passCap = (long)std::max(s.maxIt, 1) * 64 + 4096;
-  long launched = 0;
-  int passesBefore = 0;
-  for (;;) {
-    // with profiling, the first chunk of the solve is the timed sample
-    const bool timed = prof && launched == 0;
-    if (s.noGraph || timed) enqueuePasses<TM>(P, st, kPassesPerChunk, timed ? ev : nullptr, prof);
-    else DV_HIP(hipGraphLaunch(cache.exec, st));
-    launched += kPassesPerChunk;
-    DV_HIP(hipMemcpyAsync(s.stateHost, P.a.st, sizeof(ApgdState), hipMemcpyDeviceToHost, st));
-    DV_HIP(hipStreamSynchronize(st));
-    if (timed) {
-      // passes of this chunk that did work are its first (passes - before)
-      int ran = std::min(s.stateHost->passes - passesBefore, kPassesPerChunk);
-      if (prof == 4) ran = std::min(ran, 1);  // one chunk-end decision per chunk
-      for (int k = 0; k < ran; ++k) {
-        float ms = 0;
-        DV_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
-        s.stats.prof_kernel_ms += ms;
-        s.stats.prof_kernel_count += 1;
-      }
-    }
-    passesBefore = s.stateHost->passes;
-    if (s.stateHost->done) break;
-    if (launched > passCap) throw DvError{DV_ERR_DIVERGED, "APGD exceeded its pass budget"};
-  }
-  // gamma := gammaHat and the objective
-  hipLaunchKernelGGL(kFin1<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a);
-  hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.a, 3, 0);
-  hipLaunchKernelGGL(kFin3<TM>, dim3(P.gc), dim3(kTC), 0, st, P.a);
-  hipLaunchKernelGGL(kFinDecide<TM>, dim3(1), dim3(kTD), 0, st, P.a, P.gc);
-  DV_HIP(hipMemcpyAsync(s.stateHost, P.a.st, sizeof(ApgdState), hipMemcpyDeviceToHost, st));
-  DV_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace
@@ -848,16 +931,73 @@ bool GraphCache::matches(const void* k, size_t n) const {
 }
 void GraphCache::setKey(const void* k, size_t n) { key.assign((const char*)k, (const char*)k + n); }
 
+void TileSet::release() {
+  items.release();
+  bodies.release();
+  heavy.release();
+  counts.release();
+  slots.release();
+  hItems.clear();
+  hBodies.clear();
+  hSlots.clear();
+}
+
+void Engine::release() {
+  table.release();
+  active.release();
+  admit.release();
+  retire.release();
+  out.release();
+  if (outHost) (void)hipHostFree(outHost);
+  outHost = nullptr;
+  outHostCap = 0;
+  graph.reset();
+  hostTable.clear();
+  slotScene.clear();
+  nSlots = 0;
+}
+
+namespace {
+
+// Per-scene buffers and table entry.  Work buffers are sized by capacity, so
+// that a captured graph stays valid while counts drift.
 template <typename TM>
-void setupPlan(Scene& s, Plan<TM>& P, TM* WM, long RsCap) {
-  std::memset(&P, 0, sizeof(P));
-  Args<TM>& a = P.a;
-  a.dims = s.solveDims;
+void prepareScene(Scene& s, Args<TM>& a) {
+  constexpr bool kMixed = sizeof(TM) < sizeof(double);
+  if (!s.state) DV_HIP(hipMalloc(&s.state, 2 * sizeof(ApgdState)));  // by pass parity
+  const long RsCap = std::max<long>(3L * s.ncCap + s.nbil, 1);
+  const long itemsCap = (long)s.ncCap + s.nbil;
+  const long tilesCap = std::max<long>(1, (itemsCap + kTC - 1) / kTC);
+  s.partials.reserve((size_t)4 * kSlots * tilesCap);
+  s.work.reserve((size_t)5 * RsCap + 16);  // ring[4], gHat: FP64 in both solvers
+  // TM block: rn[4], r, delta, then J (per contact) and tmp (per body) of 4
+  // components 32-byte aligned, then (mixed) M^-1
+  const size_t tmElems = (size_t)6 * RsCap + 8 + 4 * ((size_t)std::max(s.ncCap, 1) + (size_t)std::max(s.nb, 1)) +
+                         (kMixed ? (size_t)s.nb + 1 : 0) + 16;
+  TM* WM;
+  if (kMixed) {
+    s.frame32.reserve((size_t)6 * std::max(s.ncCap, 1));  // n, u
+    s.work32.reserve(tmElems);
+    WM = reinterpret_cast<TM*>(s.work32.p);
+  } else {
+    s.workM64.reserve(tmElems);
+    WM = reinterpret_cast<TM*>(s.workM64.p);
+  }
+  std::memset(&a, 0, sizeof(a));
+  const double cnt = (double)((long)s.nc + s.nbil);
+  a.dims.nc = s.nc; a.dims.nbil = s.nbil; a.dims.nb = s.nb; a.dims.maxIt = s.maxIt;
+  a.dims.mu = s.mu; a.dims.tol = s.tol; a.dims.gdiff = 1.0 / (cnt * cnt);
+  a.dims.warm = s.warm ? 1 : 0; a.dims.geomMode = s.geomMode;
+  const long items = (long)s.nc + s.nbil;
+  a.nTiles = (int)((items + kTC - 1) / kTC);
+  a.active = 1;
   a.st = s.state;
+  a.frame64 = s.frame.p;
   a.cap = s.ncCap;
   a.cA = s.cA.p; a.cB = s.cB.p; a.iStart = s.iStart.p; a.inc = s.inc.p;
   a.bilIncStart = s.bilIncStart.p; a.bilInc = s.bilInc.p; a.bilA = s.bilA.p; a.bilB = s.bilB.p;
   a.heavy = s.heavy.p; a.counters = s.counters.p;
+  a.minv64 = s.minv.p;
   a.gammaIO = s.gamma.p;
   a.rFlat = s.r.p;
   double* W = s.work.p;
@@ -865,74 +1005,335 @@ void setupPlan(Scene& s, Plan<TM>& P, TM* WM, long RsCap) {
   a.gHat = W + 4 * RsCap;
   for (int k = 0; k < 4; ++k) a.rn[k] = WM + k * RsCap;
   a.r = WM + 4 * RsCap;
-  a.delta = WM + 5 * RsCap;
+  a.delta = kMixed ? WM + 5 * RsCap : nullptr;
   TM* after = WM + 6 * RsCap;
-  // J (per contact) and tmp (per body), 4 components, 32-byte aligned
   uintptr_t p = (reinterpret_cast<uintptr_t>(after) + 31) & ~uintptr_t(31);
   a.J = reinterpret_cast<V4<TM>*>(p);
   a.tmp = a.J + std::max<long>(s.ncCap, 1);
-  for (int k = 0; k < 2; ++k) {
-    a.part1[k] = s.partials.p + (long)(2 * k) * kSlots * kMaxGridC;
-    a.part3[k] = s.partials.p + (long)(2 * k + 1) * kSlots * kMaxGridC;
+  if (kMixed) {
+    a.frame = reinterpret_cast<TM*>(s.frame32.p);
+    a.minv = reinterpret_cast<TM*>(a.tmp + std::max(s.nb, 1));
+  } else {
+    a.frame = reinterpret_cast<TM*>(s.frame.p);
+    a.minv = reinterpret_cast<TM*>(s.minv.p);
   }
-  const long itemsCap = (long)s.ncCap + s.nbil;
-  // one 16-wavefront workgroup per CU (grid-stride beyond): equal work per CU
-  P.gc = std::max(1, std::min({blocksFor(itemsCap, kTC), s.numCU, kMaxGridC}));
-  P.gb = std::max(1, std::min(blocksFor(s.nb, kTB), 4096));
+  for (int k = 0; k < 2; ++k) {
+    a.part1[k] = s.partials.p + (long)(2 * k) * kSlots * tilesCap;
+    a.part3[k] = s.partials.p + (long)(2 * k + 1) * kSlots * tilesCap;
+  }
 }
 
 template <typename TM>
-size_t workElems(Scene& s, long RsCap) {
-  return (size_t)6 * RsCap + 8 + 4 * ((size_t)std::max(s.ncCap, 1) + (size_t)std::max(s.nb, 1)) +
-         (sizeof(TM) == 4 ? (size_t)s.nb + 1 : 0) + 16;
+Args<TM>& slotArgs(Engine& E, int slot) {
+  return reinterpret_cast<Args<TM>*>(E.hostTable.data())[slot];
+}
+
+// Tiles of the slots in set.hSlots, uploaded with the device slot list and
+// the heavy-body table (built on the device from the slots' counters).
+template <typename TM>
+void buildSet(Engine& E, TileSet& set, hipStream_t st) {
+  set.hItems.clear();
+  set.hBodies.clear();
+  long heavyCap = 1;
+  for (int slot : set.hSlots) {
+    const Args<TM>& a = slotArgs<TM>(E, slot);
+    for (int t = 0; t < a.nTiles; ++t) set.hItems.push_back(Tile{slot, t * kTC});
+    for (int b = 0; b < a.dims.nb; b += kTB) set.hBodies.push_back(Tile{slot, b});
+    heavyCap += 2L * std::max<long>(a.cap, 1) / kHeavyDegree + 1;
+  }
+  set.items.reserve(std::max<size_t>(set.hItems.size(), 1));
+  set.bodies.reserve(std::max<size_t>(set.hBodies.size(), 1));
+  set.heavy.reserve((size_t)heavyCap);
+  set.counts.reserve(4);
+  set.slots.reserve(std::max<size_t>(set.hSlots.size(), 1));
+  set.countsHost[0] = (int)set.hItems.size();
+  set.countsHost[1] = (int)set.hBodies.size();
+  set.countsHost[2] = 0;
+  if (!set.hItems.empty())
+    DV_HIP(hipMemcpyAsync(set.items.p, set.hItems.data(), sizeof(Tile) * set.hItems.size(),
+                          hipMemcpyHostToDevice, st));
+  if (!set.hBodies.empty())
+    DV_HIP(hipMemcpyAsync(set.bodies.p, set.hBodies.data(), sizeof(Tile) * set.hBodies.size(),
+                          hipMemcpyHostToDevice, st));
+  if (!set.hSlots.empty())
+    DV_HIP(hipMemcpyAsync(set.slots.p, set.hSlots.data(), sizeof(int) * set.hSlots.size(),
+                          hipMemcpyHostToDevice, st));
+  DV_HIP(hipMemcpyAsync(set.counts.p, set.countsHost, 3 * sizeof(int), hipMemcpyHostToDevice, st));
+  if (!set.hSlots.empty())
+    hipLaunchKernelGGL(kHeavyTable<TM>, dim3(1), dim3(1024), 0, st,
+                       reinterpret_cast<const Args<TM>*>(E.table.p), set.slots.p, (int)set.hSlots.size(),
+                       set.heavy.p, set.counts.p);
+}
+
+template <typename TM>
+Tables tablesOf(Engine& E, TileSet& set) {
+  Tables T;
+  T.items = set.items.p;
+  T.bodies = set.bodies.p;
+  T.heavy = set.heavy.p;
+  T.counts = set.counts.p;
+  T.slots = set.slots.p;
+  T.out = reinterpret_cast<ApgdState*>(E.out.p);
+  return T;
+}
+
+// Grid of the contact kernels: one 16-wavefront workgroup per CU (two with
+// several slots), never more than the tiles a single slot can hold, so that
+// the captured graph survives until a capacity grows.
+template <typename TM>
+void grids(Engine& E, const TileSet& set, int& gc, int& gb) {
+  if (E.fold) {
+    const Args<TM>& a = slotArgs<TM>(E, 0);
+    const long itemsCap = a.cap + a.dims.nbil;
+    const long tilesCap = std::max<long>(1, (itemsCap + kTC - 1) / kTC);
+    gc = (int)std::min<long>(tilesCap, E.numCU);
+    gb = (int)std::max<long>(1, std::min<long>((a.dims.nb + kTB - 1) / kTB, 4L * E.numCU));
+  } else {
+    gc = 2 * E.numCU;
+    gb = 4 * E.numCU;
+  }
+  (void)set;
+}
+
+template <typename TM>
+Plan<TM> activePlan(Engine& E) {
+  Plan<TM> P;
+  std::memset(&P, 0, sizeof(P));
+  P.A = reinterpret_cast<const Args<TM>*>(E.table.p);
+  P.T = tablesOf<TM>(E, E.active);
+  grids<TM>(E, E.active, P.gc, P.gb);
+  P.nSlots = E.nSlots;
+  P.fold = E.fold;
+  return P;
+}
+
+template <typename TM>
+void rebuildActive(Engine& E, hipStream_t st) {
+  E.active.hSlots.clear();
+  for (int i = 0; i < E.nSlots; ++i)
+    if (slotArgs<TM>(E, i).active) E.active.hSlots.push_back(i);
+  buildSet<TM>(E, E.active, st);
+}
+
+template <typename TM>
+void uploadTable(Engine& E, hipStream_t st) {
+  DV_HIP(hipMemcpyAsync(E.table.p, E.hostTable.data(), E.hostTable.size(), hipMemcpyHostToDevice, st));
+}
+
+template <typename TM>
+void startAdmitted(Engine& E, hipStream_t st) {
+  if (E.admit.hSlots.empty()) return;
+  uploadTable<TM>(E, st);
+  buildSet<TM>(E, E.admit, st);
+  const Args<TM>* A = reinterpret_cast<const Args<TM>*>(E.table.p);
+  const Tables T = tablesOf<TM>(E, E.admit);
+  int gc, gb;
+  grids<TM>(E, E.admit, gc, gb);
+  const int n = (int)E.admit.hSlots.size();
+  // setup (APGD.cu:195-223)
+  hipLaunchKernelGGL(kSetup1<TM>, dim3(gc), dim3(kTC), 0, st, A, T);
+  hipLaunchKernelGGL(kBody<TM>, dim3(gb), dim3(kTB), 0, st, A, T, 0, 0);
+  hipLaunchKernelGGL(kSetup3<TM>, dim3(gc), dim3(kTC), 0, st, A, T);
+  hipLaunchKernelGGL(kBody<TM>, dim3(gb), dim3(kTB), 0, st, A, T, 1, 0);
+  hipLaunchKernelGGL(kSetup5<TM>, dim3(gc), dim3(kTC), 0, st, A, T);
+  hipLaunchKernelGGL(kSetupDecide<TM>, dim3(n), dim3(kTD), 0, st, A, T);
+  E.admit.hSlots.clear();
+  rebuildActive<TM>(E, st);
+}
+
+template <typename TM>
+void runChunk(Engine& E, hipStream_t st, hipEvent_t* ev, int prof) {
+  const Plan<TM> P = activePlan<TM>(E);
+  if (E.noGraph || ev) {
+    enqueuePasses<TM>(P, st, kPassesPerChunk, ev, prof);
+  } else {
+    if (!E.graph.matches(&P, sizeof(P))) {
+      E.graph.reset();
+      hipGraph_t graph;
+      DV_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      enqueuePasses<TM>(P, st, kPassesPerChunk, nullptr, 0);
+      DV_HIP(hipStreamEndCapture(st, &graph));
+      hipError_t e = hipGraphInstantiate(&E.graph.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) throw HipError{e, "hipGraphInstantiate(apgd passes)"};
+      E.graph.setKey(&P, sizeof(P));
+    }
+    DV_HIP(hipGraphLaunch(E.graph.exec, st));
+  }
+  E.launched += kPassesPerChunk;
+}
+
+// gamma := gammaHat and the objective for the slots, then their statistics
+template <typename TM>
+void retireSlots(Engine& E, const std::vector<int>& slots, hipStream_t st) {
+  if (slots.empty()) return;
+  E.retire.hSlots = slots;
+  buildSet<TM>(E, E.retire, st);
+  const Args<TM>* A = reinterpret_cast<const Args<TM>*>(E.table.p);
+  const Tables T = tablesOf<TM>(E, E.retire);
+  int gc, gb;
+  grids<TM>(E, E.retire, gc, gb);
+  const int fin = E.fold ? 0 : 1;  // where the latest state sits after a chunk
+  hipLaunchKernelGGL(kFin1<TM>, dim3(gc), dim3(kTC), 0, st, A, T, fin);
+  hipLaunchKernelGGL(kBody<TM>, dim3(gb), dim3(kTB), 0, st, A, T, 3, 0);
+  hipLaunchKernelGGL(kFin3<TM>, dim3(gc), dim3(kTC), 0, st, A, T);
+  hipLaunchKernelGGL(kFinDecide<TM>, dim3((int)slots.size()), dim3(kTD), 0, st, A, T, fin);
+  ApgdState* outH = reinterpret_cast<ApgdState*>(E.outHost);
+  DV_HIP(hipMemcpyAsync(outH, E.out.p, sizeof(ApgdState) * E.nSlots, hipMemcpyDeviceToHost, st));
+  DV_HIP(hipStreamSynchronize(st));
+  for (int slot : slots) {
+    Scene& s = *E.slotScene[slot];
+    const ApgdState& S = outH[slot];
+    s.stats.iterations = S.it;
+    s.stats.backtracks = S.backtracks;
+    s.stats.residual = S.best;
+    s.stats.objective = S.objective;
+    s.stats.passes = S.passes;
+    if (s.verbose) printf("  Iterations: %d Residual: %g\n", S.it, S.best);
+    slotArgs<TM>(E, slot).active = 0;
+    E.slotScene[slot] = nullptr;
+  }
+  uploadTable<TM>(E, st);
+  rebuildActive<TM>(E, st);
+}
+
+template <typename TM>
+void openT(Engine& E, int nSlots) {
+  E.hostTable.assign(sizeof(Args<TM>) * nSlots, 0);  // every slot inactive
+  E.slotScene.assign(nSlots, nullptr);
+  E.table.reserve(sizeof(Args<TM>) * nSlots);
+}
+
+}  // namespace
+
+void engineOpen(Engine& E, int nSlots, bool mixed, int numCU, bool noGraph) {
+  if (nSlots < 1) throw DvError{DV_ERR_ARG, "an engine needs at least one slot"};
+  if (E.mixed != (mixed ? 1 : 0) || E.nSlots != nSlots) E.graph.reset();
+  E.mixed = mixed ? 1 : 0;
+  E.nSlots = nSlots;
+  E.fold = nSlots == 1 ? 1 : 0;
+  E.numCU = numCU;
+  E.noGraph = noGraph;
+  E.launched = 0;
+  E.admit.hSlots.clear();
+  if (mixed) openT<float>(E, nSlots);
+  else openT<double>(E, nSlots);
+  E.out.reserve(sizeof(ApgdState) * nSlots);
+  if (E.outHostCap < nSlots) {
+    if (E.outHost) (void)hipHostFree(E.outHost);
+    E.outHost = nullptr;
+    DV_HIP(hipHostMalloc(&E.outHost, sizeof(ApgdState) * nSlots, hipHostMallocDefault));
+    E.outHostCap = nSlots;
+  }
+}
+
+void engineAdmit(Engine& E, int slot, Scene& s) {
+  if (slot < 0 || slot >= E.nSlots || E.slotScene[slot]) throw DvError{DV_ERR_ARG, "engine slot not free"};
+  if ((long)s.nc + s.nbil == 0) throw DvError{DV_ERR_ARG, "a scene without rows needs no solve"};
+  if (E.mixed) prepareScene<float>(s, slotArgs<float>(E, slot));
+  else prepareScene<double>(s, slotArgs<double>(E, slot));
+  E.slotScene[slot] = &s;
+  E.admit.hSlots.push_back(slot);
+}
+
+void engineStartAdmitted(Engine& E, hipStream_t st) {
+  if (E.mixed) startAdmitted<float>(E, st);
+  else startAdmitted<double>(E, st);
+}
+
+void engineRunChunk(Engine& E, hipStream_t st) {
+  if (E.mixed) runChunk<float>(E, st, nullptr, 0);
+  else runChunk<double>(E, st, nullptr, 0);
+}
+
+std::vector<int> enginePoll(Engine& E, hipStream_t st) {
+  ApgdState* outH = reinterpret_cast<ApgdState*>(E.outHost);
+  DV_HIP(hipMemcpyAsync(outH, E.out.p, sizeof(ApgdState) * E.nSlots, hipMemcpyDeviceToHost, st));
+  DV_HIP(hipStreamSynchronize(st));
+  std::vector<int> done;
+  for (int i = 0; i < E.nSlots; ++i)
+    if (E.slotScene[i] && outH[i].done) done.push_back(i);
+  return done;
+}
+
+void engineRetire(Engine& E, const std::vector<int>& slots, hipStream_t st) {
+  if (E.mixed) retireSlots<float>(E, slots, st);
+  else retireSlots<double>(E, slots, st);
+}
+
+bool engineBusy(const Engine& E) {
+  for (Scene* s : E.slotScene)
+    if (s) return true;
+  return false;
+}
+
+namespace {
+
+// Admit all, run chunks until every scene is done, retire all.  With
+// profiling (single scene), the first chunk is launched on the stream with
+// events around the chosen kernel: the timed sample.
+template <typename TM>
+void solveAll(Engine& E, const std::vector<Scene*>& scenes, hipStream_t st, int prof,
+              std::vector<hipEvent_t>* profEv, dv_stats* profStats) {
+  const int n = (int)scenes.size();
+  for (int i = 0; i < n; ++i) engineAdmit(E, i, *scenes[i]);
+  startAdmitted<TM>(E, st);
+  hipEvent_t* ev = nullptr;
+  if (prof && profEv) {
+    while (profEv->size() < 2 * (size_t)kPassesPerChunk) {
+      hipEvent_t e;
+      DV_HIP(hipEventCreate(&e));
+      profEv->push_back(e);
+    }
+    ev = profEv->data();
+  }
+  int maxIt = 1;
+  for (Scene* s : scenes) maxIt = std::max(maxIt, s->maxIt);
+  const long passCap = (long)maxIt * 64 + 4096;
+  int passesBefore = 0;
+  ApgdState* outH = reinterpret_cast<ApgdState*>(E.outHost);
+  for (;;) {
+    const bool timed = ev && E.launched == 0;
+    runChunk<TM>(E, st, timed ? ev : nullptr, prof);
+    DV_HIP(hipMemcpyAsync(outH, E.out.p, sizeof(ApgdState) * E.nSlots, hipMemcpyDeviceToHost, st));
+    DV_HIP(hipStreamSynchronize(st));
+    if (timed && profStats) {
+      // passes of slot 0 in this chunk that did work are its first (passes - before)
+      int ran = std::min(outH[0].passes - passesBefore, kPassesPerChunk);
+      if (prof == 4 && E.fold) ran = std::min(ran, 1);  // one chunk-end decision per chunk
+      for (int k = 0; k < ran; ++k) {
+        float ms = 0;
+        DV_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+        profStats->prof_kernel_ms += ms;
+        profStats->prof_kernel_count += 1;
+      }
+    }
+    passesBefore = outH[0].passes;
+    bool all = true;
+    for (int i = 0; i < n && all; ++i) all = outH[i].done != 0;
+    if (all) break;
+    if (E.launched > passCap) throw DvError{DV_ERR_DIVERGED, "APGD exceeded its pass budget"};
+  }
+  std::vector<int> slots(n);
+  for (int i = 0; i < n; ++i) slots[i] = i;
+  retireSlots<TM>(E, slots, st);
+}
+
+}  // namespace
+
+void solveFusedMany(Engine& E, const std::vector<Scene*>& scenes, bool mixed, hipStream_t st, int numCU,
+                    bool noGraph) {
+  if (scenes.empty()) return;
+  engineOpen(E, (int)scenes.size(), mixed, numCU, noGraph);
+  if (mixed) solveAll<float>(E, scenes, st, 0, nullptr, nullptr);
+  else solveAll<double>(E, scenes, st, 0, nullptr, nullptr);
 }
 
 void solveFused(Scene& s, bool mixed) {
-  if (!s.state) {
-    DV_HIP(hipMalloc(&s.state, 2 * sizeof(ApgdState)));  // by pass parity
-    DV_HIP(hipMalloc(&s.solveDims, sizeof(SolveDims)));
-    DV_HIP(hipHostMalloc(&s.stateHost, sizeof(ApgdState), hipHostMallocDefault));
-    DV_HIP(hipHostMalloc(&s.dimsHost, sizeof(SolveDims), hipHostMallocDefault));
-  }
-  s.partials.reserve((size_t)4 * kSlots * kMaxGridC);
-  const double cnt = (double)((long)s.nc + s.nbil);
-  SolveDims& D = *s.dimsHost;
-  D.nc = s.nc; D.nbil = s.nbil; D.nb = s.nb; D.maxIt = s.maxIt;
-  D.mu = s.mu; D.tol = s.tol; D.gdiff = 1.0 / (cnt * cnt); D.warm = s.warm ? 1 : 0;
-  D.geomMode = s.geomMode;
-  DV_HIP(hipMemcpyAsync(s.solveDims, &D, sizeof(SolveDims), hipMemcpyHostToDevice, s.stream));
-  // work buffers sized by capacity, so that a captured graph stays valid
-  const long RsCap = std::max<long>(3L * s.ncCap + s.nbil, 1);
-  s.work.reserve((size_t)5 * RsCap + 16);  // ring[4], gHat: FP64 in both solvers
-  if (mixed) {
-    s.frame32.reserve((size_t)6 * s.ncCap);  // n, u
-    s.work32.reserve(workElems<float>(s, RsCap));
-    Plan<float> P;
-    setupPlan<float>(s, P, s.work32.p, RsCap);
-    float* minv32 = reinterpret_cast<float*>(P.a.tmp + std::max(s.nb, 1));
-    if (s.nc)
-      hipLaunchKernelGGL(toFloat, dim3(blocksFor(6L * s.ncCap, 256)), dim3(256), 0, s.stream, s.frame.p,
-                         s.frame32.p, 6L * s.ncCap);
-    hipLaunchKernelGGL(toFloat, dim3(blocksFor(s.nb, 256)), dim3(256), 0, s.stream, s.minv.p, minv32, (long)s.nb);
-    P.a.frame = s.frame32.p;
-    P.a.minv = minv32;
-    runSolve<float>(s, P, s.graphMixed);
-  } else {
-    s.workM64.reserve(workElems<double>(s, RsCap));
-    Plan<double> P;
-    setupPlan<double>(s, P, s.workM64.p, RsCap);
-    P.a.delta = nullptr;
-    P.a.frame = s.frame.p;
-    P.a.minv = s.minv.p;
-    runSolve<double>(s, P, s.graphFp64);
-  }
-  const ApgdState& S = *s.stateHost;
-  s.stats.iterations = S.it;
-  s.stats.backtracks = S.backtracks;
-  s.stats.residual = S.best;
-  s.stats.objective = S.objective;
-  s.stats.passes = S.passes;
-  if (s.verbose) printf("  Iterations: %d Residual: %g\n", S.it, S.best);
+  std::vector<Scene*> one{&s};
+  engineOpen(s.engine, 1, mixed, s.numCU, s.noGraph);
+  if (mixed) solveAll<float>(s.engine, one, s.stream, s.profKernel, &s.profEv, &s.stats);
+  else solveAll<double>(s.engine, one, s.stream, s.profKernel, &s.profEv, &s.stats);
 }
 
 }  // namespace dv

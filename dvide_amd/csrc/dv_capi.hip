@@ -6,9 +6,12 @@
 #include "dv_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <new>
 
 struct dv_scene {
   dv::Scene s;
@@ -132,7 +135,10 @@ void runSolver(Scene& s) {
     throw DvError{DV_ERR_DIVERGED, "APGD produced a non-finite residual"};
 }
 
-void step(Scene& s) {
+// First half of DoTimeStep (System.cu:544-568): collision detection and the
+// assembly of k, D and r on the scene's stream; returns whether there are
+// rows to solve.
+bool stepAssemble(Scene& s) {
   if (!s.initialized) throw DvError{DV_ERR_STATE, "DoTimeStep before initializeSystem"};
   const hipStream_t st = s.stream;
   s.stats = dv_stats{};
@@ -143,9 +149,18 @@ void step(Scene& s) {
   const bool rows = (s.nc + s.nbil) > 0;
   if (rows) buildR(s);
   DV_HIP(hipEventRecord(s.ev[2], st));
-  if (rows) runSolver(s);
+  return rows;
+}
+
+// Second half (System.cu:569-590): the update after the solve, time and
+// statistics.  The solve's results must be ordered before this on the
+// scene's stream.
+void stepFinish(Scene& s, bool rows, bool exactOrder) {
+  if (rows && !std::isfinite(s.stats.residual) && s.stats.iterations > 0)
+    throw DvError{DV_ERR_DIVERGED, "APGD produced a non-finite residual"};
+  const hipStream_t st = s.stream;
   DV_HIP(hipEventRecord(s.ev[3], st));
-  updateState(s, rows, s.solver == DV_SOLVER_APGD_PLAIN);
+  updateState(s, rows, exactOrder);
   DV_HIP(hipEventRecord(s.ev[4], st));
   DV_HIP(hipEventSynchronize(s.ev[4]));
   DV_HIP(hipGetLastError());
@@ -161,6 +176,12 @@ void step(Scene& s) {
   s.stats.time = s.time;
   if (s.verbose)
     printf("Time: %f (Exec. Time: %f), Collisions: %d\n", s.time, s.stats.ms_total, s.nc);
+}
+
+void step(Scene& s) {
+  const bool rows = stepAssemble(s);
+  if (rows) runSolver(s);
+  stepFinish(s, rows, s.solver == DV_SOLVER_APGD_PLAIN);
 }
 
 }  // namespace
@@ -212,14 +233,9 @@ int dv_scene_destroy(dv_scene* h) {
   for (auto* b : ibufs) b->release();
   s.slotOf.release(); s.slotCount.release(); s.slotStart.release(); s.slotFill.release();
   s.frame32.release(); s.work32.release();
-  s.graphFp64.reset();
-  s.graphMixed.reset();
+  s.engine.release();
   if (s.state) (void)hipFree(s.state);
-  if (s.ticket) (void)hipFree(s.ticket);
   s.workM64.release();
-  if (s.solveDims) (void)hipFree(s.solveDims);
-  if (s.stateHost) (void)hipHostFree(s.stateHost);
-  if (s.dimsHost) (void)hipHostFree(s.dimsHost);
   if (s.pinned) (void)hipHostFree(s.pinned);
   for (auto& e : s.ev)
     if (e) (void)hipEventDestroy(e);
@@ -535,6 +551,320 @@ int dv_solve(dv_scene* h) {
     dv::runSolver(s);
     return DV_OK;
   });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches and serving
+
+struct dv_batch {
+  int solver = DV_SOLVER_APGD_MIXED;
+  std::vector<dv_scene*> scenes;
+  dv::Engine engine;
+  hipStream_t stream = nullptr;
+  hipEvent_t solved = nullptr;
+  dv_batch_stats stats{};
+  std::string err;
+};
+
+struct dv_server {
+  struct Request {
+    long ticket;
+    dv_scene* scene;
+    double bytes;
+  };
+  int solver = DV_SOLVER_APGD_MIXED;
+  int maxSlots = 0;
+  double budget = 0, used = 0;
+  dv::Engine engine;
+  hipStream_t stream = nullptr;
+  hipEvent_t solved = nullptr;
+  int numCU = 0;
+  std::deque<Request> waiting;
+  std::vector<Request> slot;    // ticket < 0: free
+  std::deque<long> completed;   // not yet reported
+  long nextTicket = 1;
+  std::string err;
+};
+
+namespace dv {
+namespace {
+
+template <class H, class F>
+int guardedH(H* h, F&& fn) {
+  if (!h) return DV_ERR_ARG;
+  try {
+    h->err.clear();
+    return fn(*h);
+  } catch (const HipError& e) {
+    h->err = e.where + ": " + hipGetErrorString(e.code);
+    return DV_ERR_HIP;
+  } catch (const DvError& e) {
+    h->err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    return DV_ERR_STATE;
+  }
+}
+
+double msSince(const std::chrono::steady_clock::time_point& t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void needSolver(int solver) {
+  if (solver != DV_SOLVER_APGD_FUSED && solver != DV_SOLVER_APGD_MIXED)
+    throw DvError{DV_ERR_UNSUPPORTED, "batches and servers run the graph-driven solvers (FUSED or MIXED)"};
+}
+
+void needScene(dv_scene* h) {
+  if (!h) throw DvError{DV_ERR_ARG, "null scene"};
+  if (!h->s.initialized) throw DvError{DV_ERR_STATE, "scene not initialized"};
+}
+
+// Device memory the solver of a slot needs beyond the scene's own state:
+// the FP64 iterate ring (4 slots + gammaHat), the product vectors (4 N gamma,
+// r, delta), J, tmp, the FP32 frames and the partial sums.
+double solverBytes(const Scene& s, bool mixed) {
+  const double rows = 3.0 * std::max(s.ncCap, 1) + s.nbil;
+  const double tm = mixed ? 4 : 8;
+  return rows * (5 * 8 + 6 * tm) + std::max(s.ncCap, 1) * (4 * tm + (mixed ? 24 : 0)) +
+         std::max(s.nb, 1) * (5 * tm) + 4.0 * 8 * 8 * (1 + (rows / 3 + 1023) / 1024);
+}
+
+void batchStep(dv_batch& B) {
+  if (B.scenes.empty()) return;
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!B.stream) {
+    DV_HIP(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
+    DV_HIP(hipEventCreateWithFlags(&B.solved, hipEventDisableTiming));
+  }
+  B.stats = dv_batch_stats{};
+  std::vector<Scene*> solve;
+  std::vector<char> rows(B.scenes.size());
+  for (size_t i = 0; i < B.scenes.size(); ++i) {
+    Scene& s = B.scenes[i]->s;
+    rows[i] = stepAssemble(s);
+    if (rows[i]) {
+      DV_HIP(hipStreamWaitEvent(B.stream, s.ev[2], 0));
+      solve.push_back(&s);
+    }
+  }
+  const double msAsm = msSince(t0);
+  const auto t1 = std::chrono::steady_clock::now();
+  solveFusedMany(B.engine, solve, B.solver == DV_SOLVER_APGD_MIXED, B.stream, B.scenes[0]->s.numCU,
+                 B.scenes[0]->s.noGraph);
+  DV_HIP(hipEventRecord(B.solved, B.stream));
+  const double msSolve = msSince(t1);
+  const auto t2 = std::chrono::steady_clock::now();
+  for (size_t i = 0; i < B.scenes.size(); ++i) {
+    Scene& s = B.scenes[i]->s;
+    DV_HIP(hipStreamWaitEvent(s.stream, B.solved, 0));
+    stepFinish(s, rows[i] != 0, false);
+    B.stats.total_contacts += s.nc;
+    B.stats.total_iterations += s.stats.iterations;
+    B.stats.max_iterations = std::max(B.stats.max_iterations, s.stats.iterations);
+  }
+  B.stats.num_scenes = (int)B.scenes.size();
+  B.stats.ms_assembly = msAsm;
+  B.stats.ms_solve = msSolve;
+  B.stats.ms_update = msSince(t2);
+  B.stats.ms_total = msSince(t0);
+}
+
+void serverOpen(dv_server& S, int solver, int maxSlots, double budget) {
+  needSolver(solver);
+  if (maxSlots < 2) throw DvError{DV_ERR_ARG, "a server needs at least two slots"};
+  S.solver = solver;
+  S.maxSlots = maxSlots;
+  int dev = 0;
+  DV_HIP(hipGetDevice(&dev));
+  DV_HIP(hipDeviceGetAttribute(&S.numCU, hipDeviceAttributeMultiprocessorCount, dev));
+  if (budget <= 0) {
+    size_t freeB = 0, totalB = 0;
+    DV_HIP(hipMemGetInfo(&freeB, &totalB));
+    budget = 0.8 * (double)freeB;
+  }
+  S.budget = budget;
+  DV_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+  DV_HIP(hipEventCreateWithFlags(&S.solved, hipEventDisableTiming));
+  S.slot.assign(maxSlots, dv_server::Request{-1, nullptr, 0});
+  engineOpen(S.engine, maxSlots, solver == DV_SOLVER_APGD_MIXED, S.numCU, false);
+}
+
+bool inFlight(const dv_server& S, const dv_scene* h) {
+  for (const auto& r : S.waiting)
+    if (r.scene == h) return true;
+  for (const auto& r : S.slot)
+    if (r.ticket >= 0 && r.scene == h) return true;
+  return false;
+}
+
+// One turn of the server: launch a chunk of passes over the slots; while it
+// runs, admit waiting requests (their collision detection and assembly run
+// on their own streams, their setup is ordered after the chunk); then poll
+// the slots and complete the requests whose solves are done.
+void serverPoll(dv_server& S) {
+  if (engineBusy(S.engine)) engineRunChunk(S.engine, S.stream);
+  // admit in submission order while a slot is free and the budget allows
+  // (an idle server always admits one request)
+  bool admitted = false;
+  while (!S.waiting.empty()) {
+    int free = -1;
+    for (int k = 0; k < S.maxSlots && free < 0; ++k)
+      if (S.slot[k].ticket < 0) free = k;
+    if (free < 0) break;
+    dv_server::Request r = S.waiting.front();
+    r.bytes = solverBytes(r.scene->s, S.solver == DV_SOLVER_APGD_MIXED);
+    if (S.used + r.bytes > S.budget && (engineBusy(S.engine) || admitted)) break;
+    S.waiting.pop_front();
+    Scene& s = r.scene->s;
+    const bool rows = stepAssemble(s);
+    if (!rows) {  // nothing to solve: the update completes the request
+      stepFinish(s, false, false);
+      S.completed.push_back(r.ticket);
+      continue;
+    }
+    DV_HIP(hipStreamWaitEvent(S.stream, s.ev[2], 0));
+    engineAdmit(S.engine, free, s);
+    S.slot[free] = r;
+    S.used += r.bytes;
+    admitted = true;
+  }
+  if (admitted) engineStartAdmitted(S.engine, S.stream);
+  // retire: gamma := gammaHat, stats, then each scene's update
+  std::vector<int> done = enginePoll(S.engine, S.stream);
+  if (!done.empty()) {
+    engineRetire(S.engine, done, S.stream);
+    DV_HIP(hipEventRecord(S.solved, S.stream));
+    for (int k : done) {
+      dv_server::Request r = S.slot[k];
+      Scene& s = r.scene->s;
+      DV_HIP(hipStreamWaitEvent(s.stream, S.solved, 0));
+      stepFinish(s, true, false);
+      S.used -= r.bytes;
+      S.completed.push_back(r.ticket);
+      S.slot[k] = dv_server::Request{-1, nullptr, 0};
+    }
+  }
+}
+
+}  // namespace
+}  // namespace dv
+
+extern "C" {
+
+int dv_batch_create(dv_batch** out, int solver) {
+  if (!out) return DV_ERR_ARG;
+  *out = nullptr;
+  if (solver != DV_SOLVER_APGD_FUSED && solver != DV_SOLVER_APGD_MIXED) return DV_ERR_UNSUPPORTED;
+  try {
+    dv_batch* b = new dv_batch();
+    b->solver = solver;
+    *out = b;
+    return DV_OK;
+  } catch (...) {
+    return DV_ERR_NOMEM;
+  }
+}
+
+int dv_batch_destroy(dv_batch* b) {
+  if (!b) return DV_ERR_ARG;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  b->engine.release();
+  if (b->solved) (void)hipEventDestroy(b->solved);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return DV_OK;
+}
+
+const char* dv_batch_last_error(const dv_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int dv_batch_add(dv_batch* b, dv_scene* s) {
+  return dv::guardedH(b, [&](dv_batch& B) {
+    dv::needScene(s);
+    for (dv_scene* x : B.scenes)
+      if (x == s) throw dv::DvError{DV_ERR_ARG, "scene already in the batch"};
+    B.scenes.push_back(s);
+    return DV_OK;
+  });
+}
+
+int dv_batch_step(dv_batch* b) {
+  return dv::guardedH(b, [&](dv_batch& B) {
+    dv::batchStep(B);
+    return DV_OK;
+  });
+}
+
+int dv_batch_get_stats(const dv_batch* b, dv_batch_stats* out) {
+  if (!b || !out) return DV_ERR_ARG;
+  *out = b->stats;
+  return DV_OK;
+}
+
+int dv_server_create(dv_server** out, int solver, int max_slots, double hbm_budget) {
+  if (!out) return DV_ERR_ARG;
+  *out = nullptr;
+  dv_server* srv = new (std::nothrow) dv_server();
+  if (!srv) return DV_ERR_NOMEM;
+  const int rc = dv::guardedH(srv, [&](dv_server& S) {
+    dv::serverOpen(S, solver, max_slots, hbm_budget);
+    return DV_OK;
+  });
+  if (rc != DV_OK) {
+    g_createError = srv->err;
+    dv_server_destroy(srv);
+    return rc;
+  }
+  *out = srv;
+  return DV_OK;
+}
+
+int dv_server_destroy(dv_server* srv) {
+  if (!srv) return DV_ERR_ARG;
+  if (srv->stream) (void)hipStreamSynchronize(srv->stream);
+  srv->engine.release();
+  if (srv->solved) (void)hipEventDestroy(srv->solved);
+  if (srv->stream) (void)hipStreamDestroy(srv->stream);
+  delete srv;
+  return DV_OK;
+}
+
+const char* dv_server_last_error(const dv_server* srv) { return srv ? srv->err.c_str() : g_createError.c_str(); }
+
+int dv_server_submit(dv_server* srv, dv_scene* scene, long* ticket) {
+  return dv::guardedH(srv, [&](dv_server& S) {
+    dv::needScene(scene);
+    if (dv::inFlight(S, scene)) throw dv::DvError{DV_ERR_STATE, "scene already has a request in flight"};
+    const long t = S.nextTicket++;
+    S.waiting.push_back(dv_server::Request{t, scene, 0});
+    if (ticket) *ticket = t;
+    return DV_OK;
+  });
+}
+
+int dv_server_poll(dv_server* srv, long* done, int max_done, int* n_done) {
+  return dv::guardedH(srv, [&](dv_server& S) {
+    dv::serverPoll(S);
+    int n = 0;
+    while (!S.completed.empty() && n < max_done && done) {
+      done[n++] = S.completed.front();
+      S.completed.pop_front();
+    }
+    if (n_done) *n_done = n;
+    return DV_OK;
+  });
+}
+
+int dv_server_pending(const dv_server* srv, int* n) {
+  if (!srv || !n) return DV_ERR_ARG;
+  int k = (int)srv->waiting.size() + (int)srv->completed.size();
+  for (const auto& r : srv->slot)
+    if (r.ticket >= 0) ++k;
+  *n = k;
+  return DV_OK;
 }
 
 }  // extern "C"

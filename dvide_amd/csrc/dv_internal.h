@@ -72,6 +72,7 @@ constexpr int kSlots = 8;  // reduction slots of the solver kernels
 
 struct ApgdState;
 struct SolveDims;
+struct Scene;
 
 // An instantiated hipGraph and the launch parameters it was captured with.
 struct GraphCache {
@@ -80,6 +81,48 @@ struct GraphCache {
   void reset();
   bool matches(const void* k, size_t n) const;
   void setKey(const void* k, size_t n);
+};
+
+// A tile of solver work: rows (or bodies) [start, start + tile size) of one
+// scene of an engine's table.
+struct Tile {
+  int scene, start;
+};
+
+// Tiles of a set of scene slots: row tiles, body tiles, the heavy bodies and
+// their counts (device copies), and the slot list.
+struct TileSet {
+  DevBuf<Tile> items, bodies;
+  DevBuf<int2> heavy;
+  DevBuf<int> counts;  // [0] row tiles, [1] body tiles, [2] heavy bodies
+  DevBuf<int> slots;
+  std::vector<Tile> hItems, hBodies;
+  std::vector<int> hSlots;
+  int countsHost[4] = {};
+  void release();
+};
+
+// The graph-driven APGD solver over a table of scene slots
+// (dv_apgd_fused.hip): the device-resident table, the tile sets of the
+// active, newly admitted and retiring slots, the per-slot states the host
+// polls, and the captured graph of a chunk of passes.  Slots join (after
+// their setup) and leave (after their finalisation) between chunks.
+struct Engine {
+  int mixed = -1;       // 1: FP32 products; 0: FP64
+  int fold = 0;         // 1: single slot, decisions folded into P1
+  int nSlots = 0;
+  std::vector<char> hostTable;  // Args<TM>[nSlots]
+  std::vector<Scene*> slotScene;
+  DevBuf<char> table;
+  TileSet active, admit, retire;
+  DevBuf<char> out;     // ApgdState[nSlots]
+  void* outHost = nullptr;  // pinned copy of out
+  int outHostCap = 0;
+  GraphCache graph;
+  int numCU = 0;
+  bool noGraph = false;
+  long launched = 0;    // passes launched since the last reset
+  void release();
 };
 
 struct Scene {
@@ -150,12 +193,8 @@ struct Scene {
   DevBuf<double> workM64;  // products of the FP64 solver
   DevBuf<float> work32;  // products and matrix data of the mixed solver
   DevBuf<double> partials;
-  ApgdState* state = nullptr;      // device-resident APGD state
-  unsigned* ticket = nullptr;      // last-block counter of the solver passes
-  SolveDims* solveDims = nullptr;
-  ApgdState* stateHost = nullptr;  // pinned
-  SolveDims* dimsHost = nullptr;   // pinned
-  GraphCache graphFp64, graphMixed;
+  ApgdState* state = nullptr;  // device-resident APGD state, [2] by pass parity
+  Engine engine;               // the graph-driven solver of this scene alone
 
   // --- statistics of the last step
   dv_stats stats{};
@@ -179,6 +218,21 @@ void rowsD(Scene& s, const double* x, const double* badd, double* y);
 // solvers (dv_apgd_plain.hip, dv_apgd_fused.hip)
 void solvePlain(Scene& s);
 void solveFused(Scene& s, bool mixed);
+// one solve of several scenes in the same launches (on stream st); each
+// scene's results equal those of solveFused on it alone
+void solveFusedMany(Engine& E, const std::vector<Scene*>& scenes, bool mixed, hipStream_t st, int numCU,
+                    bool noGraph);
+// continuous batching on an engine (dv_apgd_fused.hip): slots are admitted
+// (scene assembled, setup launched) and retired (gamma written back, stats
+// filled) between chunks of passes
+void engineOpen(Engine& E, int nSlots, bool mixed, int numCU, bool noGraph);
+void engineAdmit(Engine& E, int slot, Scene& s);
+void engineStartAdmitted(Engine& E, hipStream_t st);
+void engineRunChunk(Engine& E, hipStream_t st);
+// copies the slots' states to the host; returns the active slots now done
+std::vector<int> enginePoll(Engine& E, hipStream_t st);
+void engineRetire(Engine& E, const std::vector<int>& slots, hipStream_t st);
+bool engineBusy(const Engine& E);
 
 // scans (dv_scan.hip): exclusive scan of n ints, out has n+1 entries
 void exclusiveScan(Scene& s, const int* in, int* out, int n);

@@ -33,6 +33,10 @@ EXPORTED = (
     "dv_num_bodies", "dv_num_contacts", "dv_get_stats", "dv_get_vector", "dv_set_vector",
     "dv_get_contacts", "dv_ccp_violation", "dv_assemble", "dv_schur_product", "dv_project_cone",
     "dv_solve",
+    "dv_batch_create", "dv_batch_destroy", "dv_batch_last_error", "dv_batch_add", "dv_batch_step",
+    "dv_batch_get_stats",
+    "dv_server_create", "dv_server_destroy", "dv_server_last_error", "dv_server_submit", "dv_server_poll",
+    "dv_server_pending",
 )
 
 
@@ -57,6 +61,15 @@ def gpu_visible() -> bool:
         return False
     n = ctypes.c_int(0)
     return hip.hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
+
+
+class DvBatchStats(ctypes.Structure):
+    _fields_ = [
+        ("num_scenes", ctypes.c_int), ("max_iterations", ctypes.c_int),
+        ("total_contacts", ctypes.c_long), ("total_iterations", ctypes.c_long),
+        ("ms_total", ctypes.c_double), ("ms_assembly", ctypes.c_double),
+        ("ms_solve", ctypes.c_double), ("ms_update", ctypes.c_double),
+    ]
 
 
 class DvideError(RuntimeError):
@@ -102,6 +115,19 @@ def load_library(path: str = LIB_PATH):
         "dv_schur_product": ([vp, _dp, _dp, ctypes.c_size_t], ctypes.c_int),
         "dv_project_cone": ([vp, _dp, ctypes.c_size_t, ctypes.c_double], ctypes.c_int),
         "dv_solve": ([vp], ctypes.c_int),
+        "dv_batch_create": ([ctypes.POINTER(vp), ctypes.c_int], ctypes.c_int),
+        "dv_batch_destroy": ([vp], ctypes.c_int),
+        "dv_batch_last_error": ([vp], ctypes.c_char_p),
+        "dv_batch_add": ([vp, vp], ctypes.c_int),
+        "dv_batch_step": ([vp], ctypes.c_int),
+        "dv_batch_get_stats": ([vp, ctypes.POINTER(DvBatchStats)], ctypes.c_int),
+        "dv_server_create": ([ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+        "dv_server_destroy": ([vp], ctypes.c_int),
+        "dv_server_last_error": ([vp], ctypes.c_char_p),
+        "dv_server_submit": ([vp, vp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+        "dv_server_poll": ([vp, ctypes.POINTER(ctypes.c_long), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+                           ctypes.c_int),
+        "dv_server_pending": ([vp, _ip], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -416,3 +442,102 @@ def from_scene(scene, solver: int = SOLVER_FUSED, *, tol=None, max_iterations=No
         s.addBilateralConstraintDOF(int(a), int(b), float(vel), float(t0))
     s.initializeSystem()
     return s
+
+
+class Batch:
+    """Scenes stepped together: one APGD solve of all of them in the same
+    graph launches (dv_batch_*).  Each member System keeps its own state and
+    statistics; its result equals stepping it alone with the same solver."""
+
+    def __init__(self, solver: int = SOLVER_MIXED):
+        L = load_library()
+        self._L = L
+        h = ctypes.c_void_p()
+        rc = L.dv_batch_create(ctypes.byref(h), int(solver))
+        if rc != DV_OK:
+            raise DvideError(rc, "dv_batch_create")
+        self._h = h
+        self.systems: List[System] = []
+
+    def _check(self, rc):
+        if rc != DV_OK:
+            msg = self._L.dv_batch_last_error(self._h)
+            raise DvideError(rc, msg.decode() if msg else "")
+
+    def add(self, system: "System"):
+        self._check(self._L.dv_batch_add(self._h, system._h))
+        self.systems.append(system)
+
+    def step(self):
+        self._check(self._L.dv_batch_step(self._h))
+
+    def stats(self) -> DvBatchStats:
+        st = DvBatchStats()
+        self._check(self._L.dv_batch_get_stats(self._h, ctypes.byref(st)))
+        return st
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.dv_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous batching of time-step requests over resident scenes
+    (dv_server_*): submit() queues one DoTimeStep of a System, poll() admits
+    waiting requests by free slot and HBM budget, runs one chunk of solver
+    passes and returns the tickets of completed requests."""
+
+    def __init__(self, solver: int = SOLVER_MIXED, max_slots: int = 64, hbm_budget: float = 0.0):
+        L = load_library()
+        self._L = L
+        h = ctypes.c_void_p()
+        rc = L.dv_server_create(ctypes.byref(h), int(solver), int(max_slots), float(hbm_budget))
+        if rc != DV_OK:
+            msg = L.dv_server_last_error(None)
+            raise DvideError(rc, msg.decode() if msg else "")
+        self._h = h
+        self._buf = (ctypes.c_long * 1024)()
+
+    def _check(self, rc):
+        if rc != DV_OK:
+            msg = self._L.dv_server_last_error(self._h)
+            raise DvideError(rc, msg.decode() if msg else "")
+
+    def submit(self, system: "System") -> int:
+        t = ctypes.c_long()
+        self._check(self._L.dv_server_submit(self._h, system._h, ctypes.byref(t)))
+        return t.value
+
+    def poll(self) -> List[int]:
+        n = ctypes.c_int()
+        self._check(self._L.dv_server_poll(self._h, self._buf, len(self._buf), ctypes.byref(n)))
+        return [self._buf[i] for i in range(n.value)]
+
+    def pending(self) -> int:
+        n = ctypes.c_int()
+        self._check(self._L.dv_server_pending(self._h, ctypes.byref(n)))
+        return n.value
+
+    def drain(self) -> List[int]:
+        done = []
+        while self.pending():
+            done += self.poll()
+        return done
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.dv_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -178,6 +178,60 @@ int dv_project_cone(dv_scene* s, double* g, size_t n, double mu);
 /* APGD::solve, APGD.cu:164-361, on the assembled system (no update). */
 int dv_solve(dv_scene* s);
 
+/* --- batches and serving ---------------------------------------------------
+ * No counterpart in the reference, which steps one System at a time.  Both
+ * step initialized scenes that stay resident on the device (bodies, contacts,
+ * warm-start gamma) and remain usable alone.  A scene's result is
+ * bit-identical whether it is stepped alone (dv_step with the same solver) or
+ * in a batch or a server mix: its partial sums are formed per tile of its
+ * own rows and reduced in a fixed order. */
+
+typedef struct dv_batch dv_batch;
+typedef struct dv_server dv_server;
+
+typedef struct dv_batch_stats {
+  int num_scenes;
+  int max_iterations;       /* over the scenes */
+  long total_contacts;
+  long total_iterations;
+  double ms_total;          /* host wall time of the step */
+  double ms_assembly;       /* collision detection and assembly of all scenes */
+  double ms_solve;          /* the shared APGD launches */
+  double ms_update;
+} dv_batch_stats;
+
+/* solver: DV_SOLVER_APGD_FUSED (FP64) or DV_SOLVER_APGD_MIXED */
+int dv_batch_create(dv_batch** out, int solver);
+int dv_batch_destroy(dv_batch* b);
+const char* dv_batch_last_error(const dv_batch* b);
+/* adds an initialized scene; the batch does not own it */
+int dv_batch_add(dv_batch* b, dv_scene* s);
+/* one DoTimeStep of every scene: per-scene collision detection and
+ * assembly, one APGD solve of all scenes in the same graph launches, per-scene
+ * update; each scene's dv_get_stats is filled as by dv_step */
+int dv_batch_step(dv_batch* b);
+int dv_batch_get_stats(const dv_batch* b, dv_batch_stats* out);
+
+/* A continuous-batching server of time-step requests.  Requests wait in
+ * submission order; a waiting request is admitted when a slot is free and
+ * the solver memory of the admitted scenes stays within hbm_budget bytes
+ * (<= 0: 80% of the device memory free at creation).  Its collision
+ * detection and assembly run at admission, its solve joins the slots
+ * already iterating at the next chunk of passes, and it completes when its
+ * APGD has converged (or hit maxIterations) and its update has run. */
+int dv_server_create(dv_server** out, int solver, int max_slots, double hbm_budget);
+int dv_server_destroy(dv_server* srv);
+const char* dv_server_last_error(const dv_server* srv);
+/* queues one DoTimeStep of `scene` (one request per scene in flight);
+ * *ticket identifies it in dv_server_poll */
+int dv_server_submit(dv_server* srv, dv_scene* scene, long* ticket);
+/* admits waiting requests, runs one chunk of solver passes, completes the
+ * requests whose solves are done; their tickets go to done[0..*n_done) (at
+ * most max_done; the rest are reported by later polls) */
+int dv_server_poll(dv_server* srv, long* done, int max_done, int* n_done);
+/* number of requests waiting or in flight */
+int dv_server_pending(const dv_server* srv, int* n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,117 @@
+"""Batched and served solves (dv_batch_*, dv_server_*) on the GPU.
+
+A scene's partial sums are formed per tile of its own rows and reduced in a
+fixed order, so its whole step -- contacts, iterations, gamma, v, p -- is
+bit-identical alone, in a batch and in a server mix, whatever the other
+scenes, slots or admission order.  The tests hold the mixed path to that
+bar as well as the FP64 path.
+"""
+import numpy as np
+import pytest
+
+from dvide_amd import scenes, system
+
+pytestmark = pytest.mark.gpu
+
+# (seed, target contacts): uneven sizes, one with a bilateral-driven lid
+CASES = [("pile", 21, 3000), ("pile", 22, 800), ("compaction", 2002, 0), ("pile", 24, 6000)]
+
+
+def make(kind, seed, nc):
+    if kind == "pile":
+        return scenes.pile(seed, nc, 0.03, 0.06, 0.1 + 0.1 * (seed % 4))
+    return scenes.compaction(seed, n=3000, width=1.2, lid=scenes.LID_BILATERAL)
+
+
+def bits_equal(x, y):
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    if x.shape != y.shape:
+        return False
+    nx, ny = np.isnan(x), np.isnan(y)
+    return np.array_equal(nx, ny) and np.array_equal(x[~nx].view(np.uint64), y[~ny].view(np.uint64))
+
+
+def same_state(a, b, tag):
+    sa, sb = a.stats(), b.stats()
+    assert sa.num_contacts == sb.num_contacts, tag
+    assert sa.iterations == sb.iterations, (tag, sa.iterations, sb.iterations)
+    assert sa.residual == sb.residual, tag
+    for vec in (system.VEC_GAMMA, system.VEC_V, system.VEC_P):
+        assert bits_equal(a.vector(vec), b.vector(vec)), (tag, vec)
+
+
+def systems(solver, tol=1e-5):
+    return [system.from_scene(make(*c), solver, tol=tol, max_iterations=5000) for c in CASES]
+
+
+@pytest.mark.parametrize("solver", [system.SOLVER_FUSED, system.SOLVER_MIXED])
+def test_batch_equals_alone(gpu, solver):
+    alone = systems(solver)
+    together = systems(solver)
+    batch = system.Batch(solver)
+    for g in together:
+        batch.add(g)
+    for k in range(3):
+        for g in alone:
+            g.step()
+        batch.step()
+        st = batch.stats()
+        assert st.num_scenes == len(CASES)
+        assert st.total_contacts == sum(g.stats().num_contacts for g in alone)
+        for i, (a, b) in enumerate(zip(alone, together)):
+            same_state(a, b, (k, i))
+    batch.close()
+
+
+@pytest.mark.parametrize("max_slots,budget", [(8, 0.0), (2, 0.0), (4, 1.0)])
+def test_server_mix_equals_alone(gpu, max_slots, budget):
+    """Slots fewer than requests and a budget that admits one scene at a time
+    change only when each request runs, not its result."""
+    solver = system.SOLVER_MIXED
+    alone = systems(solver)
+    served = systems(solver)
+    srv = system.Server(solver, max_slots=max_slots, hbm_budget=budget)
+    for k in range(2):
+        for g in alone:
+            g.step()
+        tickets = {srv.submit(g): i for i, g in enumerate(served)}
+        done = srv.drain()
+        assert sorted(done) == sorted(tickets)
+        for i, (a, b) in enumerate(zip(alone, served)):
+            same_state(a, b, (k, i))
+    srv.close()
+
+
+def test_server_staggered_arrivals(gpu):
+    """Requests submitted while others iterate join at chunk boundaries."""
+    solver = system.SOLVER_FUSED
+    alone = systems(solver)
+    served = systems(solver)
+    for g in alone:
+        g.step()
+    srv = system.Server(solver, max_slots=4)
+    done = []
+    for g in served:
+        srv.submit(g)
+        done += srv.poll()
+    done += srv.drain()
+    assert len(done) == len(served)
+    for i, (a, b) in enumerate(zip(alone, served)):
+        same_state(a, b, i)
+    srv.close()
+
+
+def test_server_rejects_a_second_request_in_flight(gpu):
+    g = system.from_scene(make("pile", 21, 3000), system.SOLVER_MIXED)
+    srv = system.Server(system.SOLVER_MIXED, max_slots=2)
+    srv.submit(g)
+    with pytest.raises(system.DvideError):
+        srv.submit(g)
+    srv.drain()
+    srv.close()
+
+
+def test_batch_rejects_the_plain_solver(gpu):
+    with pytest.raises(system.DvideError):
+        system.Batch(system.SOLVER_PLAIN)

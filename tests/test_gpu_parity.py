@@ -4,10 +4,11 @@ Bit-exact: the contact set, its normals, gaps and frames, k, b and r, the
 Schur product N x, the cone projection, and the whole plain FP64 APGD
 trajectory (gamma, iterations, residual, v, p) -- the plain path performs
 the oracle's operations in the oracle's order.
-Tolerance-level: the persistent solvers, which reorder reductions and use
-N y = (1+b) N g_k - b N g_{k-1}: FP64 gamma within 1e-6 relative and mixed
-FP32/FP64 gamma within 1e-3 relative of the FP64 oracle, both at a residual
-<= tol.
+Tolerance-level: the graph-driven solvers, which reorder reductions and use
+N y = (1+b) N g_k - b N g_{k-1}: the FP64 solver's gamma within 1e-6 of the
+oracle's at tolerance 1e-6; the mixed solver's early iterates within 2e-5 of
+the oracle's, and its solution at tolerance 1e-4 as close to the exact one
+as the oracle's at that tolerance (gamma itself is not unique on piles).
 """
 import numpy as np
 import pytest
@@ -103,24 +104,70 @@ def test_plain_trajectory_bit_exact(gpu):
         assert bits_equal(o.vec(wo), g.vector(wg)), wg
 
 
-@pytest.mark.parametrize("solver,tol,tol_gamma", [(system.SOLVER_FUSED, 1e-6, 1e-6),
-                                                   (system.SOLVER_MIXED, 1e-4, 1e-3)])
-def test_fused_solvers_match_oracle(gpu, solver, tol, tol_gamma):
-    """Graph-driven solvers against the oracle's FP64 APGD on the same step."""
+def test_fp64_solver_matches_oracle(gpu):
+    """Graph-driven FP64 APGD against the oracle's on the same step: the
+    products are summed in another order, so the trajectories agree to
+    rounding and the solutions to well below the tolerance."""
     sc = scenes.compaction(2002, n=4000, width=1.5, lid=scenes.LID_BILATERAL)
-    o, g = both(sc, solver=solver, tol=tol, max_iterations=20000)
+    o, g = both(sc, solver=system.SOLVER_FUSED, tol=1e-6, max_iterations=20000)
+    o.step()
+    g.step()
+    st = g.stats()
+    assert st.num_contacts == o.num_contacts
+    assert st.residual < 1e-6
+    # the residual the solver reports is the reference's getResidual of the
+    # gamma it returns, evaluated by the oracle in FP64
+    res_o = o.residual_of(g.gamma)
+    assert abs(res_o - st.residual) <= 1e-3 * 1e-6 + 1e-9 * abs(st.residual), (res_o, st.residual)
+    assert rel(g.gamma, o.vec(oracle.GAMMA)) < 1e-6
+    assert rel(g.v, o.vec(oracle.V)) < 1e-6
+
+
+def _free(sc):
+    return np.repeat(sc.fixed == 0, 3)
+
+
+def test_mixed_solver_early_iterations_track_oracle(gpu):
+    """The mixed solver performs the reference's APGD steps: after a fixed
+    number of iterations its gamma differs from the FP64 oracle's only by
+    the binary32 rounding of D, M^-1 and the products."""
+    sc = scenes.compaction(2002, n=4000, width=1.5, lid=scenes.LID_BILATERAL)
+    for m in (1, 5, 10):
+        o, g = both(sc, solver=system.SOLVER_MIXED, tol=0.0, max_iterations=m)
+        o.step()
+        g.step()
+        assert g.stats().iterations == o.iterations == m
+        assert rel(g.gamma, o.vec(oracle.GAMMA)) < 2e-5, m
+
+
+def test_mixed_solver_at_matched_residual(gpu):
+    """Mixed FP32/FP64 APGD at tolerance 1e-4 against the FP64 oracle.
+
+    gamma is not unique here (N is rank-deficient), D^T gamma is: the test
+    compares the residual the oracle evaluates in FP64 on the returned gamma,
+    and the velocities and contact forces of the free bodies against the
+    oracle's exact solution (tolerance 1e-9): the mixed solution must be as
+    close to it as the oracle's own solution at the same tolerance (within a
+    factor 3), and within 2e-2 relative in absolute terms."""
+    sc = scenes.compaction(2002, n=4000, width=1.5, lid=scenes.LID_BILATERAL)
+    tol = 1e-4
+    exact = oracle.OracleSystem(sc, tol=1e-9, max_iterations=100000)
+    exact.step()
+    assert exact.residual < 1e-9
+    o, g = both(sc, solver=system.SOLVER_MIXED, tol=tol, max_iterations=20000)
     o.step()
     g.step()
     st = g.stats()
     assert st.num_contacts == o.num_contacts
     assert st.residual < tol
-    # the residual the solver reports is the reference's getResidual of the
-    # gamma it returns, evaluated by the oracle in FP64
-    res_o = o.residual_of(g.gamma)
-    assert abs(res_o - st.residual) <= 1e-3 * tol + 1e-9 * abs(st.residual), (res_o, st.residual)
-    go, gg = o.vec(oracle.GAMMA), g.gamma
-    assert rel(gg, go) < tol_gamma, rel(gg, go)
-    assert rel(g.v, o.vec(oracle.V)) < tol_gamma
+    assert o.residual_of(g.gamma) < 1.1 * tol
+    free = _free(sc)
+    v_star, f_star = exact.vec(oracle.V)[free], exact.vec(oracle.F_CONTACT)[free]
+    for vec_o, vec_g, star in ((oracle.V, g.v, v_star), (oracle.F_CONTACT, g.f_contact, f_star)):
+        err_mixed = rel(vec_g[free], star)
+        err_oracle = rel(o.vec(vec_o)[free], star)
+        assert err_mixed < max(3 * err_oracle, 1e-6), (vec_o, err_mixed, err_oracle)
+        assert err_mixed < 2e-2, (vec_o, err_mixed)
 
 
 def test_ball_on_box_gpu(gpu):
