@@ -384,6 +384,20 @@ int dv_num_bodies(const dv_scene* h, int* n) {
   return DV_OK;
 }
 
+int dv_get_bodies(const dv_scene* h, double* mass, double* geom, int* fixed, int* family, size_t n) {
+  if (!h) return DV_ERR_ARG;
+  const Scene& s = h->s;
+  const size_t m = std::min(n, s.hMass.size());
+  for (size_t i = 0; i < m; ++i) {
+    if (mass) mass[i] = s.hMass[i];
+    if (fixed) fixed[i] = s.hFixed[i];
+    if (family) family[i] = s.hFamily[i];
+    if (geom)
+      for (int q = 0; q < 3; ++q) geom[3 * i + q] = s.hGeom[3 * i + q];
+  }
+  return DV_OK;
+}
+
 int dv_num_contacts(const dv_scene* h, int* nc) {
   if (!h || !nc) return DV_ERR_ARG;
   *nc = h->s.nc;
@@ -464,6 +478,95 @@ int dv_get_contacts(dv_scene* h, uint32_t* idA, uint32_t* idB, double* nap, doub
       if (loc)
         for (int q = 0; q < 3; ++q) loc[3 * c + q] = lc[(size_t)q * s.ncCap + c];
     }
+    return DV_OK;
+  });
+}
+
+int dv_get_jacobian(dv_scene* h, int* I, int* J, double* V, size_t cap, size_t* nnz, int* rows, int* cols) {
+  return dv::guarded(h, [&](Scene& s) {
+    if (!s.initialized) return dv::fail(s, DV_ERR_STATE, "not initialized");
+    const int nc = s.nc, nbil = (int)s.hBilA.size();
+    size_t bilNnz = 0;
+    for (int q = 0; q < nbil; ++q) bilNnz += s.hBilB[q] >= 0 ? 2 : 1;
+    const size_t total = bilNnz + 18 * (size_t)nc;
+    if (nnz) *nnz = total;
+    if (rows) *rows = 3 * nc + nbil;
+    if (cols) *cols = 3 * s.nb;
+    if (!I || !J || !V) return DV_OK;
+    if (cap < total) return dv::fail(s, DV_ERR_ARG, "buffers smaller than nnz");
+    std::vector<double> fr(9 * (size_t)std::max(s.ncCap, 1));
+    std::vector<int> a(std::max(nc, 1)), b(std::max(nc, 1));
+    if (nc) {
+      DV_HIP(hipMemcpyAsync(fr.data(), s.frame.p, sizeof(double) * 9 * s.ncCap, hipMemcpyDeviceToHost, s.stream));
+      DV_HIP(hipMemcpyAsync(a.data(), s.cA.p, sizeof(int) * nc, hipMemcpyDeviceToHost, s.stream));
+      DV_HIP(hipMemcpyAsync(b.data(), s.cB.p, sizeof(int) * nc, hipMemcpyDeviceToHost, s.stream));
+      DV_HIP(hipStreamSynchronize(s.stream));
+    }
+    size_t o = 0;
+    for (int q = 0; q < nbil; ++q) {  // System.cu:622-638
+      I[o] = q; J[o] = s.hBilA[q]; V[o] = 1.0; ++o;
+      if (s.hBilB[q] >= 0) { I[o] = q; J[o] = s.hBilB[q]; V[o] = -1.0; ++o; }
+    }
+    for (int c = 0; c < nc; ++c) {
+      for (int rr = 0; rr < 3; ++rr) {
+        const int row = nbil + 3 * c + rr;
+        for (int q = 0; q < 3; ++q) {
+          I[o] = row; J[o] = 3 * a[c] + q; V[o] = fr[(size_t)(3 * rr + q) * s.ncCap + c]; ++o;
+        }
+        for (int q = 0; q < 3; ++q) {
+          I[o] = row; J[o] = 3 * b[c] + q; V[o] = -fr[(size_t)(3 * rr + q) * s.ncCap + c]; ++o;
+        }
+      }
+    }
+    return DV_OK;
+  });
+}
+
+int dv_contact_forces(dv_scene* h, double* normal, double* friction, size_t n) {
+  return dv::guarded(h, [&](Scene& s) {
+    if (!normal || !friction) return dv::fail(s, DV_ERR_ARG, "null buffer");
+    if (n < (size_t)s.nc) return dv::fail(s, DV_ERR_ARG, "buffers smaller than the contact count");
+    const int nc = s.nc;
+    if (nc == 0) return DV_OK;
+    std::vector<double> fr(9 * (size_t)s.ncCap), g((size_t)s.gammaLen);
+    DV_HIP(hipMemcpyAsync(fr.data(), s.frame.p, sizeof(double) * 9 * s.ncCap, hipMemcpyDeviceToHost, s.stream));
+    DV_HIP(hipMemcpyAsync(g.data(), s.gamma.p, sizeof(double) * g.size(), hipMemcpyDeviceToHost, s.stream));
+    DV_HIP(hipStreamSynchronize(s.stream));
+    const double h = s.h;
+    for (int c = 0; c < nc; ++c) {
+      double F[9];
+      for (int k = 0; k < 9; ++k) F[k] = fr[(size_t)k * s.ncCap + c];
+      const size_t i0 = 3 * (size_t)c;  // no bilateral offset (System.cu:1279)
+      const double g0 = i0 < g.size() ? g[i0] : 0, g1 = i0 + 1 < g.size() ? g[i0 + 1] : 0,
+                   g2 = i0 + 2 < g.size() ? g[i0 + 2] : 0;
+      for (int q = 0; q < 3; ++q) {
+        normal[3 * c + q] = g0 * F[q] / h;
+        friction[3 * c + q] = (g1 * F[3 + q] + g2 * F[6 + q]) / h;
+      }
+    }
+    return DV_OK;
+  });
+}
+
+int dv_energies(dv_scene* h, double out[3]) {
+  return dv::guarded(h, [&](Scene& s) {
+    if (!s.initialized) return dv::fail(s, DV_ERR_STATE, "not initialized");
+    const size_t n3 = 3 * (size_t)s.nb;
+    std::vector<double> p(n3), v(n3), f(n3), mi(s.nb);
+    DV_HIP(hipMemcpyAsync(p.data(), s.p.p, sizeof(double) * n3, hipMemcpyDeviceToHost, s.stream));
+    DV_HIP(hipMemcpyAsync(v.data(), s.v.p, sizeof(double) * n3, hipMemcpyDeviceToHost, s.stream));
+    DV_HIP(hipMemcpyAsync(f.data(), s.f.p, sizeof(double) * n3, hipMemcpyDeviceToHost, s.stream));
+    DV_HIP(hipMemcpyAsync(mi.data(), s.minv.p, sizeof(double) * s.nb, hipMemcpyDeviceToHost, s.stream));
+    DV_HIP(hipStreamSynchronize(s.stream));
+    double pe = 0, ke = 0;
+    for (size_t i = 0; i < n3; ++i) {
+      pe += f[i] * p[i];
+      const double m = mi[i / 3] != 0 ? 1.0 / mi[i / 3] : 0.0;  // multiplyByMass, System.cu:1411-1417
+      ke += v[i] * (m * v[i]);
+    }
+    out[0] = -pe;
+    out[1] = 0.5 * ke;
+    out[2] = 0.0;
     return DV_OK;
   });
 }

@@ -30,9 +30,9 @@ VEC_P, VEC_V, VEC_GAMMA, VEC_F_CONTACT, VEC_R, VEC_K, VEC_B = range(7)
 EXPORTED = (
     "dv_scene_create", "dv_scene_destroy", "dv_last_error", "dv_add_bodies", "dv_add_bilateral_dof",
     "dv_set_bins_per_axis", "dv_set_param", "dv_get_param", "dv_initialize", "dv_step",
-    "dv_num_bodies", "dv_num_contacts", "dv_get_stats", "dv_get_vector", "dv_set_vector",
+    "dv_num_bodies", "dv_get_bodies", "dv_num_contacts", "dv_get_stats", "dv_get_vector", "dv_set_vector",
     "dv_get_contacts", "dv_ccp_violation", "dv_assemble", "dv_schur_product", "dv_project_cone",
-    "dv_solve",
+    "dv_solve", "dv_get_jacobian", "dv_contact_forces", "dv_energies",
     "dv_batch_create", "dv_batch_destroy", "dv_batch_last_error", "dv_batch_add", "dv_batch_step",
     "dv_batch_get_stats",
     "dv_server_create", "dv_server_destroy", "dv_server_last_error", "dv_server_submit", "dv_server_poll",
@@ -105,6 +105,7 @@ def load_library(path: str = LIB_PATH):
         "dv_initialize": ([vp], ctypes.c_int),
         "dv_step": ([vp], ctypes.c_int),
         "dv_num_bodies": ([vp, _ip], ctypes.c_int),
+        "dv_get_bodies": ([vp, _dp, _dp, _ip, _ip, ctypes.c_size_t], ctypes.c_int),
         "dv_num_contacts": ([vp, _ip], ctypes.c_int),
         "dv_get_stats": ([vp, ctypes.POINTER(DvStats)], ctypes.c_int),
         "dv_get_vector": ([vp, ctypes.c_int, _dp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
@@ -115,6 +116,10 @@ def load_library(path: str = LIB_PATH):
         "dv_schur_product": ([vp, _dp, _dp, ctypes.c_size_t], ctypes.c_int),
         "dv_project_cone": ([vp, _dp, ctypes.c_size_t, ctypes.c_double], ctypes.c_int),
         "dv_solve": ([vp], ctypes.c_int),
+        "dv_get_jacobian": ([vp, _ip, _ip, _dp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _ip, _ip],
+                            ctypes.c_int),
+        "dv_contact_forces": ([vp, _dp, _dp, ctypes.c_size_t], ctypes.c_int),
+        "dv_energies": ([vp, _dp], ctypes.c_int),
         "dv_batch_create": ([ctypes.POINTER(vp), ctypes.c_int], ctypes.c_int),
         "dv_batch_destroy": ([vp], ctypes.c_int),
         "dv_batch_last_error": ([vp], ctypes.c_char_p),
@@ -385,6 +390,75 @@ class System:
         self._check(self._L.dv_get_contacts(self._h, a.ctypes.data_as(_up), b.ctypes.data_as(_up),
                                             nap.ctypes.data_as(_dp), loc.ctypes.data_as(_dp), nc))
         return a, b, nap, loc
+
+    def _bodies(self):
+        self._flush_pending()
+        n = self.numBodies
+        mass = np.zeros(n)
+        geom = np.zeros((n, 3))
+        fixed = np.zeros(n, np.int32)
+        family = np.zeros(n, np.int32)
+        self._check(self._L.dv_get_bodies(self._h, mass.ctypes.data_as(_dp), geom.ctypes.data_as(_dp),
+                                          fixed.ctypes.data_as(_ip), family.ctypes.data_as(_ip), n))
+        return mass, geom, fixed, family
+
+    def masses(self):
+        return self._bodies()[0]
+
+    def geometry(self):
+        return self._bodies()[1]
+
+    def fixed(self):
+        return self._bodies()[2]
+
+    def families(self):
+        return self._bodies()[3]
+
+    def inverse_mass(self):
+        """Diagonal of System::mass (1/m, 0 for a fixed body), one per body."""
+        mass, _, fixed, _ = self._bodies()
+        return np.where(fixed != 0, 0.0, 1.0 / np.where(mass != 0, mass, 1.0))
+
+    def jacobian(self):
+        """System::D as row-ordered COO (I, J, V, rows, cols)."""
+        nnz = ctypes.c_size_t()
+        rows = ctypes.c_int()
+        cols = ctypes.c_int()
+        self._check(self._L.dv_get_jacobian(self._h, None, None, None, 0, ctypes.byref(nnz), ctypes.byref(rows),
+                                            ctypes.byref(cols)))
+        n = nnz.value
+        I = np.zeros(n, np.int32)
+        J = np.zeros(n, np.int32)
+        V = np.zeros(n, np.float64)
+        if n:
+            self._check(self._L.dv_get_jacobian(self._h, I.ctypes.data_as(_ip), J.ctypes.data_as(_ip),
+                                                V.ctypes.data_as(_dp), n, None, None, None))
+        return I, J, V, rows.value, cols.value
+
+    def contactForcePerCollision(self):
+        """System::calculateContactForcePerCollision: (normal, friction), each (nc, 3)."""
+        nc = self.numCollisions
+        nrm = np.zeros((nc, 3))
+        fr = np.zeros((nc, 3))
+        self._check(self._L.dv_contact_forces(self._h, nrm.ctypes.data_as(_dp), fr.ctypes.data_as(_dp), nc))
+        return nrm, fr
+
+    def _energies(self):
+        out = np.zeros(3)
+        self._check(self._L.dv_energies(self._h, out.ctypes.data_as(_dp)))
+        return out
+
+    def getPotentialEnergy(self):
+        return float(self._energies()[0])
+
+    def getKineticEnergy(self):
+        return float(self._energies()[1])
+
+    def getStrainEnergy(self):
+        return float(self._energies()[2])
+
+    def getTotalEnergy(self):
+        return float(self._energies().sum())
 
     def assemble(self) -> int:
         nc = ctypes.c_int()

@@ -138,6 +138,9 @@ int dv_step(dv_scene* s);
 /* --- data access (System's public device vectors) ------------------------- */
 
 int dv_num_bodies(const dv_scene* s, int* n);
+/* System::bodies (Body::getMass, getGeometry, isFixed, getCollisionFamily):
+ * copies of the first min(n, count) bodies' fields; any pointer may be NULL */
+int dv_get_bodies(const dv_scene* s, double* mass, double* geom, int* fixed, int* family, size_t n);
 int dv_num_contacts(const dv_scene* s, int* nc);
 int dv_get_stats(const dv_scene* s, dv_stats* out);
 
@@ -156,6 +159,22 @@ int dv_get_contacts(dv_scene* s, uint32_t* idA, uint32_t* idB, double* nap, doub
 /* System::getCCPViolation, System.cu:1605-1643: min(gamma_n,0), min((Dv+b)_n,0),
  * gamma_n . (Dv+b)_n and the 2-norm of min(mu gamma_n - |gamma_t|, 0). */
 int dv_ccp_violation(dv_scene* s, double out[4]);
+
+/* System::D (System.cu:1351-1386) as row-ordered COO: the bilateral rows
+ * first (+1 at dofA, -1 at dofB), then per contact the rows n, u, v, each with
+ * body A's three columns (+) before body B's (-).  *nnz, *rows, *cols get the
+ * sizes; I, J, V (cap entries, may be NULL to query) get the entries.  Feeds
+ * System::exportMatrices (System.cu:1912-1930). */
+int dv_get_jacobian(dv_scene* s, int* I, int* J, double* V, size_t cap, size_t* nnz, int* rows, int* cols);
+
+/* System::calculateContactForcePerCollision (System.cu:1259-1289): per contact
+ * gamma_n n / h and (gamma_u u + gamma_v v) / h, 3 values each (nc entries).
+ * Like the reference it reads gamma at 3 i without the bilateral offset. */
+int dv_contact_forces(dv_scene* s, double* normal, double* friction, size_t nc);
+
+/* System::getPotentialEnergy, getKineticEnergy, getStrainEnergy
+ * (System.cu:1645-1666): -f.p, v.(M v)/2 and 0 (no elastic bodies here). */
+int dv_energies(dv_scene* s, double out[3]);
 
 /* --- hot-path pieces, exposed for parity tests ---------------------------- */
 
