@@ -13,13 +13,12 @@
 //       bodies with many contacts are reduced by a whole workgroup;
 //   P3 (contacts): (N gamma_new)_c = F_c (tmp_A - tmp_B), gamma_new . N gamma_new
 //       and the residual's infinity norm (getResidual, APGD.cu:90-102);
-// followed by each scene's decision: a fixed-order reduction of its partial
-// sums, the Lipschitz test (APGD.cu:236-264) and either a backtracking step
-// or the acceptance logic (APGD.cu:266-332).  For a single scene the
-// decision is folded into the next P1: every workgroup reduces the same
-// partials in the same order, reaches the same state, and workgroup 0
-// publishes it; a one-wavefront kernel decides the last pass of a chunk.  A
-// batch runs one decision kernel per pass with a wavefront per scene.
+// followed by each scene's decision (one wavefront per scene): a
+// fixed-order reduction of its partial sums, the Lipschitz test
+// (APGD.cu:236-264) and either a backtracking step or the acceptance logic
+// (APGD.cu:266-332).  (Folding the decision into the next P1, every
+// workgroup deciding redundantly, measured slower on MI355X: 25.8 us for
+// that P1 against 15.3 + 7.3 us for P1 and the decision kernel.)
 //
 // Work is cut into tiles of one scene (kTC rows or kTB bodies); a scene's
 // partial sums are kept per tile and reduced in tile order, so a scene gives
@@ -68,7 +67,6 @@ struct ApgdState {
   double sum[2 * kSlots];
   int it, bt, backtracks, restart;
   int cur, prev, nxt;
-  int pending;  // partials of a pass are waiting for their decision (folded mode)
   int hatSrc, done, converged, passes;
 };
 
@@ -524,7 +522,7 @@ __global__ __launch_bounds__(kTD) void kSetupDecide(const Args<TM>* __restrict__
     S.objective = 0;
     for (int k = 0; k < 2 * kSlots; ++k) S.sum[k] = 0;
     S.it = 0; S.bt = 0; S.backtracks = 0; S.restart = 1;
-    S.cur = 0; S.prev = 0; S.nxt = 1; S.pending = 0; S.hatSrc = -1;  // -1: gHat (ones)
+    S.cur = 0; S.prev = 0; S.nxt = 1; S.hatSrc = -1;  // -1: gHat (ones)
     S.done = a.dims.maxIt <= 0 ? 1 : 0;
     S.converged = 0;
     S.passes = 0;
@@ -594,41 +592,16 @@ __device__ void decideStep(ApgdState& S, const double* tot, const SolveDims& D) 
   if (S.it >= D.maxIt) S.done = 1;
 }
 
-// Folded decision (single scene): P1 of parity par reads st[par] and, if a
-// pass is pending, the partials part1/part3[par ^ 1] of that pass; wavefront
-// 0 of every workgroup applies the decision to its copy, workgroup 0
-// publishes the result in st[par ^ 1] for P2, P3 and the next pass.
-template <typename TM>
-__device__ void passState(const Args<TM>& a, int par, ApgdState& sh) {
-  constexpr bool kMixed = sizeof(TM) < sizeof(double);
-  if (threadIdx.x < 64) {
-    ApgdState S = a.st[par];
-    if (S.pending && !S.done) {
-      double tot[2 * kSlots];
-      reduceAll(a.part1[par ^ 1], a.part3[par ^ 1], a.nTiles, tot);
-      decideStep<kMixed>(S, tot, a.dims);
-    }
-    S.pending = S.done ? 0 : 1;
-    if (threadIdx.x == 0) {
-      sh = S;
-      if (blockIdx.x == 0) a.st[par ^ 1] = S;
-    }
-  }
-  __syncthreads();
-}
-
 // ---------------------------------------------------------------------------
 // one pass (parity par): the state of the pass is st[par ^ 1]
 
 template <typename TM>
-__global__ __launch_bounds__(kTC) void kP1(const Args<TM>* __restrict__ A, Tables T, int par, int fold) {
+__global__ __launch_bounds__(kTC) void kP1(const Args<TM>* __restrict__ A, Tables T, int par) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
-  __shared__ ApgdState shS;
-  if (fold) passState(A[0], par, shS);
   TILE_LOOP(T) {
     const Tile tile = T.items[ti_];
     const Args<TM>& a = A[tile.scene];
-    const ApgdState S = fold ? shS : a.st[par ^ 1];
+    const ApgdState S = a.st[par ^ 1];
     if (S.done) continue;
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
@@ -752,28 +725,22 @@ __global__ __launch_bounds__(kTC) void kP3(const Args<TM>* __restrict__ A, Table
   }
 }
 
-// Decision of the pass of parity par for every scene (one wavefront each).
-// Batches (fold 0) decide every pass: state st[par ^ 1] -> st[par].  A
-// single scene (fold 1) decides here only the last pass of a chunk, in
-// place, so that the next chunk's first P1 has nothing pending.  Both
-// record the state in T.out for the host.
+// Decision of the pass of parity par for every slot (one wavefront each):
+// state st[par ^ 1] -> st[par], recorded in T.out for the host.  A slot that
+// is done copies its state forward.
 template <typename TM>
-__global__ __launch_bounds__(kTD) void kDecide(const Args<TM>* __restrict__ A, Tables T, int par, int fold) {
+__global__ __launch_bounds__(kTD) void kDecide(const Args<TM>* __restrict__ A, Tables T, int par) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   const Args<TM>& a = A[blockIdx.x];  // one workgroup per slot
   if (!a.active) return;
-  const int src = par ^ 1;
-  const int dst = fold ? src : par;
-  ApgdState S = a.st[src];
-  const bool run = fold ? (S.pending && !S.done) : !S.done;
-  if (run) {
+  ApgdState S = a.st[par ^ 1];
+  if (!S.done) {
     double tot[2 * kSlots];
     reduceAll(a.part1[par], a.part3[par], a.nTiles, tot);
     decideStep<kMixed>(S, tot, a.dims);
   }
-  S.pending = 0;
   if (threadIdx.x == 0) {
-    a.st[dst] = S;
+    a.st[par] = S;
     T.out[blockIdx.x] = S;
   }
 }
@@ -892,29 +859,24 @@ template <typename TM>
 struct Plan {
   const Args<TM>* A;
   Tables T;
-  int gc, gb, nSlots, fold;
+  int gc, gb, nSlots;
 };
 
-// One chunk of n (even) passes, pass k with parity k & 1.  A batch decides
-// after every pass; a single scene decides inside P1 and once more at the
-// chunk end.  With `ev` (stream launches only: events recorded inside a
-// captured graph cannot be timed), events bracket kernel `prof` of pass k
-// (ev[2k], ev[2k+1]); with the folded decision, prof 4 times the chunk-end
-// decision (ev[0], ev[1]).
+// One chunk of n (even) passes, pass k with parity k & 1.  With `ev` (stream
+// launches only: events recorded inside a captured graph cannot be timed),
+// events bracket kernel `prof` of pass k (ev[2k], ev[2k+1]).
 template <typename TM>
 void enqueuePasses(const Plan<TM>& P, hipStream_t st, int n, hipEvent_t* ev, int prof) {
   for (int k = 0; k < n; ++k) {
     const int par = k & 1;
     for (int kern = 1; kern <= 4; ++kern) {
-      if (kern == 4 && P.fold && k != n - 1) continue;
       const bool timed = ev && prof == kern;
-      const int e = (kern == 4 && P.fold) ? 0 : 2 * k;
-      if (timed) DV_HIP(hipEventRecord(ev[e], st));
-      if (kern == 1) hipLaunchKernelGGL(kP1<TM>, dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par, P.fold);
+      if (timed) DV_HIP(hipEventRecord(ev[2 * k], st));
+      if (kern == 1) hipLaunchKernelGGL(kP1<TM>, dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
       if (kern == 2) hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.A, P.T, 2, par ^ 1);
       if (kern == 3) hipLaunchKernelGGL(kP3<TM>, dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
-      if (kern == 4) hipLaunchKernelGGL(kDecide<TM>, dim3(P.nSlots), dim3(kTD), 0, st, P.A, P.T, par, P.fold);
-      if (timed) DV_HIP(hipEventRecord(ev[e + 1], st));
+      if (kern == 4) hipLaunchKernelGGL(kDecide<TM>, dim3(P.nSlots), dim3(kTD), 0, st, P.A, P.T, par);
+      if (timed) DV_HIP(hipEventRecord(ev[2 * k + 1], st));
     }
   }
 }
@@ -1082,7 +1044,7 @@ Tables tablesOf(Engine& E, TileSet& set) {
 // the captured graph survives until a capacity grows.
 template <typename TM>
 void grids(Engine& E, const TileSet& set, int& gc, int& gb) {
-  if (E.fold) {
+  if (E.nSlots == 1) {
     const Args<TM>& a = slotArgs<TM>(E, 0);
     const long itemsCap = a.cap + a.dims.nbil;
     const long tilesCap = std::max<long>(1, (itemsCap + kTC - 1) / kTC);
@@ -1103,7 +1065,6 @@ Plan<TM> activePlan(Engine& E) {
   P.T = tablesOf<TM>(E, E.active);
   grids<TM>(E, E.active, P.gc, P.gb);
   P.nSlots = E.nSlots;
-  P.fold = E.fold;
   return P;
 }
 
@@ -1173,7 +1134,7 @@ void retireSlots(Engine& E, const std::vector<int>& slots, hipStream_t st) {
   const Tables T = tablesOf<TM>(E, E.retire);
   int gc, gb;
   grids<TM>(E, E.retire, gc, gb);
-  const int fin = E.fold ? 0 : 1;  // where the latest state sits after a chunk
+  const int fin = 1;  // a chunk's last pass (parity 1) decides into st[1]
   hipLaunchKernelGGL(kFin1<TM>, dim3(gc), dim3(kTC), 0, st, A, T, fin);
   hipLaunchKernelGGL(kBody<TM>, dim3(gb), dim3(kTB), 0, st, A, T, 3, 0);
   hipLaunchKernelGGL(kFin3<TM>, dim3(gc), dim3(kTC), 0, st, A, T);
@@ -1211,7 +1172,6 @@ void engineOpen(Engine& E, int nSlots, bool mixed, int numCU, bool noGraph) {
   if (E.mixed != (mixed ? 1 : 0) || E.nSlots != nSlots) E.graph.reset();
   E.mixed = mixed ? 1 : 0;
   E.nSlots = nSlots;
-  E.fold = nSlots == 1 ? 1 : 0;
   E.numCU = numCU;
   E.noGraph = noGraph;
   E.launched = 0;
@@ -1299,8 +1259,7 @@ void solveAll(Engine& E, const std::vector<Scene*>& scenes, hipStream_t st, int 
     DV_HIP(hipStreamSynchronize(st));
     if (timed && profStats) {
       // passes of slot 0 in this chunk that did work are its first (passes - before)
-      int ran = std::min(outH[0].passes - passesBefore, kPassesPerChunk);
-      if (prof == 4 && E.fold) ran = std::min(ran, 1);  // one chunk-end decision per chunk
+      const int ran = std::min(outH[0].passes - passesBefore, kPassesPerChunk);
       for (int k = 0; k < ran; ++k) {
         float ms = 0;
         DV_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));

@@ -109,7 +109,6 @@ struct TileSet {
 // their setup) and leave (after their finalisation) between chunks.
 struct Engine {
   int mixed = -1;       // 1: FP32 products; 0: FP64
-  int fold = 0;         // 1: single slot, decisions folded into P1
   int nSlots = 0;
   std::vector<char> hostTable;  // Args<TM>[nSlots]
   std::vector<Scene*> slotScene;
