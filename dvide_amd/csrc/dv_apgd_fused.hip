@@ -96,7 +96,6 @@ struct Args {
   SolveDims dims;
   int nTiles;           // row tiles of this solve (partial sums per slot)
   int active;           // the slot holds a scene being solved
-  ApgdState* st;        // [2], by pass parity
   const double* frame64;  // assembled frames: 9 SoA arrays n, u, v of stride cap
   TM* frame;            // frames the passes read: frame64's n (FP64) or an FP32 n, u copy
   long cap;             // contact capacity: row-vector stride
@@ -133,8 +132,14 @@ struct Tables {
   const int2* heavy;    // (slot, body) of bodies reduced by a workgroup
   const int* counts;    // [0] row tiles, [1] body tiles, [2] heavy bodies
   const int* slots;     // the set's slots (one decide workgroup each)
+  ApgdState* st;        // [slot][2]: the solver state by pass parity
   ApgdState* out;       // per slot: the state after the latest decision
+  int single;           // one slot: tile k is rows k*kTC.. of slot 0 (no table read)
 };
+
+// state of slot `slot` at parity k: at a fixed offset from the kernel
+// argument, so reading it does not wait for a table load
+#define STATE(T, slot, k) ((T).st[2 * (slot) + (k)])
 
 // row offsets: contact c component k -> k*cap + c; bilateral b -> 3*cap + b
 #define ROW(a, k, c) ((long)(k) * (a).cap + (c))
@@ -288,6 +293,13 @@ __device__ __forceinline__ void zero(double (&acc)[kSlots]) {
 // loop of a workgroup over the row tiles (uniform across the workgroup)
 #define TILE_LOOP(T) for (int ti_ = blockIdx.x, nt_ = (T).counts[0]; ti_ < nt_; ti_ += gridDim.x)
 
+__device__ __forceinline__ Tile rowTile(const Tables& T, int k) {
+  return T.single ? Tile{0, k * kTC} : T.items[k];
+}
+__device__ __forceinline__ Tile bodyTile(const Tables& T, int k) {
+  return T.single ? Tile{0, k * kTB} : T.bodies[k];
+}
+
 // partial-sum slots of a tile
 #define TILE_PART(p, tile) ((p) + (long)((tile).start / kTC) * kSlots)
 
@@ -300,7 +312,7 @@ template <typename TM>
 __global__ __launch_bounds__(kTC) void kSetup1(const Args<TM>* __restrict__ A, Tables T) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   TILE_LOOP(T) {
-    const Tile tile = T.items[ti_];
+    const Tile tile = rowTile(T, ti_);
     const Args<TM>& a = A[tile.scene];
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil, n3 = 3L * D.nc;
@@ -343,8 +355,8 @@ __global__ __launch_bounds__(kTC) void kSetup1(const Args<TM>* __restrict__ A, T
 // (gamma_0), 2 = ring[nxt] (gamma_new of the pass whose state is st[sIdx]),
 // 3 = gHat
 template <typename TM>
-__device__ __forceinline__ const double* bodySource(const Args<TM>& a, int which, int sIdx) {
-  return which == 0 ? a.ring[1] : which == 1 ? a.ring[0] : which == 2 ? a.ring[a.st[sIdx].nxt] : a.gHat;
+__device__ __forceinline__ const double* bodySource(const Args<TM>& a, int which, const ApgdState& S) {
+  return which == 0 ? a.ring[1] : which == 1 ? a.ring[0] : which == 2 ? a.ring[S.nxt] : a.gHat;
 }
 
 template <typename TM>
@@ -361,12 +373,13 @@ __device__ __forceinline__ TM bodyMinv(const Args<TM>& a, int g, int which) {
 template <typename TM>
 __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tables T, int which, int sIdx) {
   for (int ti = blockIdx.x, nt = T.counts[1]; ti < nt; ti += gridDim.x) {
-    const Tile tile = T.bodies[ti];
+    const Tile tile = bodyTile(T, ti);
+    const ApgdState& S = STATE(T, tile.scene, sIdx);
+    if (which == 2 && S.done) continue;
     const Args<TM>& a = A[tile.scene];
-    if (which == 2 && a.st[sIdx].done) continue;
     const int g = tile.start + threadIdx.x;
     if (g >= a.dims.nb) continue;
-    const double* X = bodySource(a, which, sIdx);
+    const double* X = bodySource(a, which, S);
     const long bil0 = 3L * a.cap;
     const int k0 = a.iStart[g], k1 = a.iStart[g + 1];
     if (k1 - k0 > kHeavyDegree) continue;  // reduced below by a workgroup
@@ -406,8 +419,9 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
   }
   for (int hq = blockIdx.x, nh = T.counts[2]; hq < nh; hq += gridDim.x) {
     const int2 ref = T.heavy[hq];
+    const ApgdState& S = STATE(T, ref.x, sIdx);
+    if (which == 2 && S.done) continue;
     const Args<TM>& a = A[ref.x];
-    if (which == 2 && a.st[sIdx].done) continue;
     const int g = ref.y;
     const TM mi = bodyMinv(a, g, which);
     if (mi == (TM)0) {
@@ -418,7 +432,7 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
       }
       continue;
     }
-    const double* X = bodySource(a, which, sIdx);
+    const double* X = bodySource(a, which, S);
     const long bil0 = 3L * a.cap;
     const int k0 = a.iStart[g], k1 = a.iStart[g + 1];
     double acc[kSlots];
@@ -454,7 +468,7 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
 template <typename TM>
 __global__ __launch_bounds__(kTC) void kSetup3(const Args<TM>* __restrict__ A, Tables T) {
   TILE_LOOP(T) {
-    const Tile tile = T.items[ti_];
+    const Tile tile = rowTile(T, ti_);
     const Args<TM>& a = A[tile.scene];
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
@@ -483,7 +497,7 @@ __global__ __launch_bounds__(kTC) void kSetup3(const Args<TM>* __restrict__ A, T
 template <typename TM>
 __global__ __launch_bounds__(kTC) void kSetup5(const Args<TM>* __restrict__ A, Tables T) {
   TILE_LOOP(T) {
-    const Tile tile = T.items[ti_];
+    const Tile tile = rowTile(T, ti_);
     const Args<TM>& a = A[tile.scene];
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
@@ -526,8 +540,8 @@ __global__ __launch_bounds__(kTD) void kSetupDecide(const Args<TM>* __restrict__
     S.done = a.dims.maxIt <= 0 ? 1 : 0;
     S.converged = 0;
     S.passes = 0;
-    a.st[0] = S;
-    a.st[1] = S;
+    STATE(T, slot, 0) = S;
+    STATE(T, slot, 1) = S;
     T.out[slot] = S;
   }
 }
@@ -599,10 +613,10 @@ template <typename TM>
 __global__ __launch_bounds__(kTC) void kP1(const Args<TM>* __restrict__ A, Tables T, int par) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   TILE_LOOP(T) {
-    const Tile tile = T.items[ti_];
-    const Args<TM>& a = A[tile.scene];
-    const ApgdState S = a.st[par ^ 1];
+    const Tile tile = rowTile(T, ti_);
+    const ApgdState S = STATE(T, tile.scene, par ^ 1);
     if (S.done) continue;
+    const Args<TM>& a = A[tile.scene];
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
     const long q = (long)tile.start + threadIdx.x;
@@ -673,10 +687,10 @@ template <typename TM>
 __global__ __launch_bounds__(kTC) void kP3(const Args<TM>* __restrict__ A, Tables T, int par) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   TILE_LOOP(T) {
-    const Tile tile = T.items[ti_];
-    const Args<TM>& a = A[tile.scene];
-    const ApgdState& S = a.st[par ^ 1];
+    const Tile tile = rowTile(T, ti_);
+    const ApgdState& S = STATE(T, tile.scene, par ^ 1);
     if (S.done) continue;
+    const Args<TM>& a = A[tile.scene];
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
     const long q = (long)tile.start + threadIdx.x;
@@ -733,14 +747,14 @@ __global__ __launch_bounds__(kTD) void kDecide(const Args<TM>* __restrict__ A, T
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   const Args<TM>& a = A[blockIdx.x];  // one workgroup per slot
   if (!a.active) return;
-  ApgdState S = a.st[par ^ 1];
+  ApgdState S = STATE(T, blockIdx.x, par ^ 1);
   if (!S.done) {
     double tot[2 * kSlots];
     reduceAll(a.part1[par], a.part3[par], a.nTiles, tot);
     decideStep<kMixed>(S, tot, a.dims);
   }
   if (threadIdx.x == 0) {
-    a.st[par] = S;
+    STATE(T, blockIdx.x, par) = S;
     T.out[blockIdx.x] = S;
   }
 }
@@ -752,12 +766,12 @@ __global__ __launch_bounds__(kTD) void kDecide(const Args<TM>* __restrict__ A, T
 template <typename TM>
 __global__ __launch_bounds__(kTC) void kFin1(const Args<TM>* __restrict__ A, Tables T, int fin) {
   TILE_LOOP(T) {
-    const Tile tile = T.items[ti_];
+    const Tile tile = rowTile(T, ti_);
     const Args<TM>& a = A[tile.scene];
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
     const long q = (long)tile.start + threadIdx.x;
-    const int hatSrc = a.st[fin].hatSrc;
+    const int hatSrc = STATE(T, tile.scene, fin).hatSrc;
     const double* hatV = hatSrc >= 0 ? a.ring[hatSrc] : a.gHat;
     double acc[kSlots];
     zero(acc);
@@ -786,7 +800,7 @@ __global__ __launch_bounds__(kTC) void kFin1(const Args<TM>* __restrict__ A, Tab
 template <typename TM>
 __global__ __launch_bounds__(kTC) void kFin3(const Args<TM>* __restrict__ A, Tables T) {
   TILE_LOOP(T) {
-    const Tile tile = T.items[ti_];
+    const Tile tile = rowTile(T, ti_);
     const Args<TM>& a = A[tile.scene];
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
@@ -817,9 +831,9 @@ __global__ __launch_bounds__(kTD) void kFinDecide(const Args<TM>* __restrict__ A
   double tot[2 * kSlots];
   reduceAll(a.part1[0], a.part3[0], a.nTiles, tot);
   if (threadIdx.x == 0) {
-    ApgdState S = a.st[fin];
+    ApgdState S = STATE(T, slot, fin);
     S.objective = 0.5 * tot[kSlots + 1] + tot[0];
-    a.st[fin] = S;
+    STATE(T, slot, fin) = S;
     T.out[slot] = S;
   }
 }
@@ -910,6 +924,7 @@ void Engine::release() {
   admit.release();
   retire.release();
   out.release();
+  states.release();
   if (outHost) (void)hipHostFree(outHost);
   outHost = nullptr;
   outHostCap = 0;
@@ -926,7 +941,6 @@ namespace {
 template <typename TM>
 void prepareScene(Scene& s, Args<TM>& a) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
-  if (!s.state) DV_HIP(hipMalloc(&s.state, 2 * sizeof(ApgdState)));  // by pass parity
   const long RsCap = std::max<long>(3L * s.ncCap + s.nbil, 1);
   const long itemsCap = (long)s.ncCap + s.nbil;
   const long tilesCap = std::max<long>(1, (itemsCap + kTC - 1) / kTC);
@@ -953,7 +967,6 @@ void prepareScene(Scene& s, Args<TM>& a) {
   const long items = (long)s.nc + s.nbil;
   a.nTiles = (int)((items + kTC - 1) / kTC);
   a.active = 1;
-  a.st = s.state;
   a.frame64 = s.frame.p;
   a.cap = s.ncCap;
   a.cA = s.cA.p; a.cB = s.cB.p; a.iStart = s.iStart.p; a.inc = s.inc.p;
@@ -1036,6 +1049,8 @@ Tables tablesOf(Engine& E, TileSet& set) {
   T.counts = set.counts.p;
   T.slots = set.slots.p;
   T.out = reinterpret_cast<ApgdState*>(E.out.p);
+  T.st = reinterpret_cast<ApgdState*>(E.states.p);
+  T.single = E.nSlots == 1 ? 1 : 0;
   return T;
 }
 
@@ -1179,6 +1194,7 @@ void engineOpen(Engine& E, int nSlots, bool mixed, int numCU, bool noGraph) {
   if (mixed) openT<float>(E, nSlots);
   else openT<double>(E, nSlots);
   E.out.reserve(sizeof(ApgdState) * nSlots);
+  E.states.reserve(2 * sizeof(ApgdState) * nSlots);
   if (E.outHostCap < nSlots) {
     if (E.outHost) (void)hipHostFree(E.outHost);
     E.outHost = nullptr;

@@ -234,7 +234,6 @@ int dv_scene_destroy(dv_scene* h) {
   s.slotOf.release(); s.slotCount.release(); s.slotStart.release(); s.slotFill.release();
   s.frame32.release(); s.work32.release();
   s.engine.release();
-  if (s.state) (void)hipFree(s.state);
   s.workM64.release();
   if (s.pinned) (void)hipHostFree(s.pinned);
   for (auto& e : s.ev)

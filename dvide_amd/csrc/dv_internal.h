@@ -115,6 +115,7 @@ struct Engine {
   DevBuf<char> table;
   TileSet active, admit, retire;
   DevBuf<char> out;     // ApgdState[nSlots]
+  DevBuf<char> states;  // ApgdState[nSlots][2], by pass parity
   void* outHost = nullptr;  // pinned copy of out
   int outHostCap = 0;
   GraphCache graph;
@@ -192,7 +193,6 @@ struct Scene {
   DevBuf<double> workM64;  // products of the FP64 solver
   DevBuf<float> work32;  // products and matrix data of the mixed solver
   DevBuf<double> partials;
-  ApgdState* state = nullptr;  // device-resident APGD state, [2] by pass parity
   Engine engine;               // the graph-driven solver of this scene alone
 
   // --- statistics of the last step
