@@ -5,8 +5,8 @@
  * Solver.cuh:17-46, APGD.cuh:62-67, Body.cuh:28-99, CollisionDetector.cuh:48-103);
  * it has no C ABI (DanVIApi.h:7-20 defines export macros that are applied to
  * nothing).  Each entry point below names the reference member it replaces.
- * The C++ drop-in surface (dvide/System.h and friends) is a thin layer over
- * these functions.
+ * The C++ drop-in surface (include/dvide/System.hpp: System, Body, Solver,
+ * APGD, CollisionDetector) is a header-only layer over these functions.
  *
  * Conventions
  *   - every function returns int: 0 on success, a negative DV_ERR_* code on

@@ -96,3 +96,18 @@ def test_compaction_lid_modes():
     assert len(bil.bil_dofs) == 3 and not bil.params["warm_start"]
     assert list(bil.bil_dofs[:, 0]) == [3 * lid, 3 * lid + 1, 3 * lid + 2]
     assert list(bil.bil_info[:, 0]) == [0.0, -0.1, 0.0]
+
+
+DRIVER = os.path.join(ROOT, "tests", "cpp", "bin", "one_ball")
+
+
+def test_cpp_driver_is_built_against_the_facade():
+    assert os.path.exists(DRIVER), "the C++ driver is built by make -C dvide_amd"
+    out = subprocess.run(["ldd", DRIVER], capture_output=True, text=True).stdout
+    assert "libdvide_hip.so" in out
+
+
+@pytest.mark.skipif(system.gpu_visible(), reason="checks the behaviour on a host without a GPU")
+def test_cpp_driver_fails_loudly_without_a_gpu():
+    r = subprocess.run([DRIVER], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "no HIP device" in r.stderr
