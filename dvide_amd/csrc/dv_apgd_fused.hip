@@ -609,9 +609,34 @@ __device__ void decideStep(ApgdState& S, const double* tot, const SolveDims& D) 
 // ---------------------------------------------------------------------------
 // one pass (parity par): the state of the pass is st[par ^ 1]
 
-template <typename TM>
-__global__ __launch_bounds__(kTC) void kP1(const Args<TM>* __restrict__ A, Tables T, int par) {
+// Frame of a contact from its loaded components: FP64 stores n (u, v are
+// rebuilt bit-identically, System.cu:753-762), FP32 stores n and u.
+__device__ __forceinline__ void frameFrom(const double (&c)[6], int mode, double (&F)[9]) {
+  const d3 n = mk3(c[0], c[1], c[2]);
+  d3 u, v;
+  contactFrame(n, mode, &u, &v);
+  F[0] = n.x; F[1] = n.y; F[2] = n.z;
+  F[3] = u.x; F[4] = u.y; F[5] = u.z;
+  F[6] = v.x; F[7] = v.y; F[8] = v.z;
+}
+__device__ __forceinline__ void frameFrom(const float (&c)[6], int, float (&F)[9]) {
+#pragma unroll
+  for (int k = 0; k < 6; ++k) F[k] = c[k];
+  F[6] = F[1] * F[5] - F[2] * F[4];  // v = n x u
+  F[7] = F[2] * F[3] - F[0] * F[5];
+  F[8] = F[0] * F[4] - F[1] * F[3];
+}
+
+// The contact kernels take R rows (contacts) per thread: a workgroup of
+// kTC / R threads covers one tile of kTC rows, and every thread issues the
+// loads of all its rows before computing, for more memory-level parallelism
+// per wavefront.  The per-tile partial sums keep their layout.
+
+template <typename TM, int R>
+__global__ __launch_bounds__(kTC / R) void kP1(const Args<TM>* __restrict__ A, Tables T, int par) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
+  constexpr int NT = kTC / R;
+  constexpr int NF = kMixed ? 6 : 3;  // stored frame components
   TILE_LOOP(T) {
     const Tile tile = rowTile(T, ti_);
     const ApgdState S = STATE(T, tile.scene, par ^ 1);
@@ -619,37 +644,54 @@ __global__ __launch_bounds__(kTC) void kP1(const Args<TM>* __restrict__ A, Table
     const Args<TM>& a = A[tile.scene];
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
-    const long q = (long)tile.start + threadIdx.x;
+    const double* Gk = a.ring[S.cur];
+    const double* Gkm1 = a.ring[S.prev];
+    const TM* Nk = a.rn[S.cur];
+    const TM* Nkm1 = a.rn[S.prev];
+    double* Gn = a.ring[S.nxt];
+    const bool mom = !S.restart;  // momentum term present
+    // loads of all rows of the thread
+    double gk[R][3], gm[R][3], rv[R][3];
+    TM nk[R][3], nm[R][3], fc[R][6];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const long q = (long)tile.start + threadIdx.x + u * NT;
+      const bool valid = q < items, bil = q >= D.nc;
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr) {
+        const bool on = valid && (rr == 0 || !bil);
+        const long f = bil ? BROW(a, q - D.nc) : ROW(a, rr, q);
+        gk[u][rr] = on ? Gk[f] : 0.0;
+        rv[u][rr] = on ? (double)a.r[f] : 0.0;
+        nk[u][rr] = on ? Nk[f] : (TM)0;
+        gm[u][rr] = (on && mom) ? Gkm1[f] : 0.0;
+        nm[u][rr] = (on && mom) ? Nkm1[f] : (TM)0;
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) fc[u][k] = (valid && !bil && k < NF) ? a.frame[k * a.cap + q] : (TM)0;
+    }
+    const double bp = 1.0 + S.beta, bm = -S.beta;
+    const double t = S.t;
     double acc[kSlots];
     zero(acc);
-    if (q < items) {
-      const double* Gk = a.ring[S.cur];
-      const double* Gkm1 = a.ring[S.prev];
-      const TM* Nk = a.rn[S.cur];
-      const TM* Nkm1 = a.rn[S.prev];
-      double* Gn = a.ring[S.nxt];
-      const double bp = 1.0 + S.beta, bm = -S.beta;
-      const double t = S.t;
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const long q = (long)tile.start + threadIdx.x + u * NT;
+      if (q >= items) continue;
       const bool bil = q >= D.nc;
       const int nr = bil ? 1 : 3;
-      double y[3] = {0, 0, 0}, ny[3] = {0, 0, 0}, gk[3] = {0, 0, 0}, gg[3] = {0, 0, 0}, gn[3] = {0, 0, 0},
-             rv[3] = {0, 0, 0};
-      long o[3] = {0, 0, 0};
+      double y[3] = {0, 0, 0}, ny[3] = {0, 0, 0}, gg[3] = {0, 0, 0}, gn[3] = {0, 0, 0};
 #pragma unroll
       for (int rr = 0; rr < 3; ++rr) {
         if (rr < nr) {
-          o[rr] = bil ? BROW(a, q - D.nc) : ROW(a, rr, q);
-          const long f = o[rr];
-          gk[rr] = Gk[f];
-          rv[rr] = (double)a.r[f];
-          if (S.restart) {
-            y[rr] = gk[rr];
-            ny[rr] = (double)Nk[f];
+          if (mom) {
+            y[rr] = bp * gk[u][rr] + bm * gm[u][rr];
+            ny[rr] = bp * (double)nk[u][rr] + bm * (double)nm[u][rr];
           } else {
-            y[rr] = bp * gk[rr] + bm * Gkm1[f];
-            ny[rr] = bp * (double)Nk[f] + bm * (double)Nkm1[f];
+            y[rr] = gk[u][rr];
+            ny[rr] = (double)nk[u][rr];
           }
-          gg[rr] = 1.0 * ny[rr] + 1.0 * rv[rr];
+          gg[rr] = 1.0 * ny[rr] + 1.0 * rv[u][rr];
           gn[rr] = 1.0 * y[rr] + (-t) * gg[rr];
         }
       }
@@ -657,35 +699,37 @@ __global__ __launch_bounds__(kTC) void kP1(const Args<TM>* __restrict__ A, Table
 #pragma unroll
       for (int rr = 0; rr < 3; ++rr) {
         if (rr < nr) {
-          const long f = o[rr];
+          const long f = bil ? BROW(a, q - D.nc) : ROW(a, rr, q);
           Gn[f] = gn[rr];
           const double dy = gn[rr] - y[rr];
           if (kMixed) {
             a.delta[f] = (TM)dy;
             acc[1] += dy * ny[rr];
           } else {
-            acc[0] += gn[rr] * rv[rr];
+            acc[0] += gn[rr] * rv[u][rr];
             acc[1] += y[rr] * ny[rr];
-            acc[2] += y[rr] * rv[rr];
+            acc[2] += y[rr] * rv[u][rr];
             acc[3] += gg[rr] * dy;
           }
           acc[4] += dy * dy;
-          acc[5] += gg[rr] * (gn[rr] - gk[rr]);
+          acc[5] += gg[rr] * (gn[rr] - gk[u][rr]);
         }
       }
       if (!bil) {
         TM F[9];
-        ldFrame(a, (int)q, D.geomMode, F);
+        frameFrom(fc[u], D.geomMode, F);
         storeJ(a, (int)q, F, (TM)gn[0], (TM)gn[1], (TM)gn[2]);
       }
     }
-    writePartials<kTC>(acc, TILE_PART(a.part1[par], tile));
+    writePartials<NT>(acc, TILE_PART(a.part1[par], tile));
   }
 }
 
-template <typename TM>
-__global__ __launch_bounds__(kTC) void kP3(const Args<TM>* __restrict__ A, Tables T, int par) {
+template <typename TM, int R>
+__global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, Tables T, int par) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
+  constexpr int NT = kTC / R;
+  constexpr int NF = kMixed ? 6 : 3;
   TILE_LOOP(T) {
     const Tile tile = rowTile(T, ti_);
     const ApgdState& S = STATE(T, tile.scene, par ^ 1);
@@ -693,12 +737,40 @@ __global__ __launch_bounds__(kTC) void kP3(const Args<TM>* __restrict__ A, Table
     const Args<TM>& a = A[tile.scene];
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
-    const long q = (long)tile.start + threadIdx.x;
+    const double* Gn = a.ring[S.nxt];
+    TM* Nn = a.rn[S.nxt];
+    // loads of all contact rows of the thread, then the tmp gathers
+    double g[R][3];
+    TM dl[R][3], rv[R][3], fc[R][6];
+    int ca[R], cb[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const long q = (long)tile.start + threadIdx.x + u * NT;
+      const bool on = q < D.nc;
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr) {
+        const long f = ROW(a, rr, q);
+        g[u][rr] = on ? Gn[f] : 0.0;
+        rv[u][rr] = on ? a.r[f] : (TM)0;
+        dl[u][rr] = (on && kMixed) ? a.delta[f] : (TM)0;
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) fc[u][k] = (on && k < NF) ? a.frame[k * a.cap + q] : (TM)0;
+      ca[u] = on ? a.cA[q] : 0;
+      cb[u] = on ? a.cB[q] : 0;
+    }
+    V4<TM> tA[R], tB[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      tA[u] = a.tmp[ca[u]];
+      tB[u] = a.tmp[cb[u]];
+    }
     double acc[kSlots];
     zero(acc);
-    if (q < items) {
-      const double* Gn = a.ring[S.nxt];
-      TM* Nn = a.rn[S.nxt];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const long q = (long)tile.start + threadIdx.x + u * NT;
+      if (q >= items) continue;
       if (q >= D.nc) {
         const long f = BROW(a, q - D.nc);
         const TM nv = bilRow(a, (int)(q - D.nc));
@@ -716,26 +788,25 @@ __global__ __launch_bounds__(kTC) void kP3(const Args<TM>* __restrict__ A, Table
         }
         acc[7] = fmax(acc[7], e);
       } else {
-        const int c = (int)q;
         TM F[9];
-        ldFrame(a, c, D.geomMode, F);
-        TM n0, n1, n2;
-        contactRows(a, c, F, n0, n1, n2);
-        const long f0 = ROW(a, 0, c), f1 = ROW(a, 1, c), f2 = ROW(a, 2, c);
-        Nn[f0] = n0; Nn[f1] = n1; Nn[f2] = n2;
-        const double g0 = Gn[f0], g1 = Gn[f1], g2 = Gn[f2];
+        frameFrom(fc[u], D.geomMode, F);
+        const TM d0 = tA[u].x - tB[u].x, d1 = tA[u].y - tB[u].y, d2 = tA[u].z - tB[u].z;
+        const TM n0 = F[0] * d0 + F[1] * d1 + F[2] * d2;
+        const TM n1 = F[3] * d0 + F[4] * d1 + F[5] * d2;
+        const TM n2 = F[6] * d0 + F[7] * d1 + F[8] * d2;
+        Nn[ROW(a, 0, q)] = n0; Nn[ROW(a, 1, q)] = n1; Nn[ROW(a, 2, q)] = n2;
+        const double g0 = g[u][0], g1 = g[u][1], g2 = g[u][2];
         if (kMixed)
-          acc[6] += (double)a.delta[f0] * (double)n0 + (double)a.delta[f1] * (double)n1 +
-                    (double)a.delta[f2] * (double)n2;
+          acc[6] += (double)dl[u][0] * (double)n0 + (double)dl[u][1] * (double)n1 + (double)dl[u][2] * (double)n2;
         else
           acc[6] += g0 * (double)n0 + g1 * (double)n1 + g2 * (double)n2;
-        const double w0 = (double)n0 + (double)a.r[f0];
-        const double w1 = (double)n1 + (double)a.r[f1];
-        const double w2 = (double)n2 + (double)a.r[f2];
+        const double w0 = (double)n0 + (double)rv[u][0];
+        const double w1 = (double)n1 + (double)rv[u][1];
+        const double w2 = (double)n2 + (double)rv[u][2];
         acc[7] = fmax(acc[7], residualContact(g0, g1, g2, w0, w1, w2, D.mu, D.gdiff));
       }
     }
-    writePartials<kTC>(acc, TILE_PART(a.part3[par], tile));
+    writePartials<NT>(acc, TILE_PART(a.part3[par], tile));
   }
 }
 
@@ -873,8 +944,19 @@ template <typename TM>
 struct Plan {
   const Args<TM>* A;
   Tables T;
-  int gc, gb, nSlots;
+  int gc, gb, nSlots, rpt;  // rpt: rows per thread of the contact kernels
 };
+
+template <typename TM>
+void launchP1(const Plan<TM>& P, hipStream_t st, int par) {
+  if (P.rpt == 2) hipLaunchKernelGGL((kP1<TM, 2>), dim3(P.gc), dim3(kTC / 2), 0, st, P.A, P.T, par);
+  else hipLaunchKernelGGL((kP1<TM, 1>), dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
+}
+template <typename TM>
+void launchP3(const Plan<TM>& P, hipStream_t st, int par) {
+  if (P.rpt == 2) hipLaunchKernelGGL((kP3<TM, 2>), dim3(P.gc), dim3(kTC / 2), 0, st, P.A, P.T, par);
+  else hipLaunchKernelGGL((kP3<TM, 1>), dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
+}
 
 // One chunk of n (even) passes, pass k with parity k & 1.  With `ev` (stream
 // launches only: events recorded inside a captured graph cannot be timed),
@@ -886,9 +968,9 @@ void enqueuePasses(const Plan<TM>& P, hipStream_t st, int n, hipEvent_t* ev, int
     for (int kern = 1; kern <= 4; ++kern) {
       const bool timed = ev && prof == kern;
       if (timed) DV_HIP(hipEventRecord(ev[2 * k], st));
-      if (kern == 1) hipLaunchKernelGGL(kP1<TM>, dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
+      if (kern == 1) launchP1<TM>(P, st, par);
       if (kern == 2) hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.A, P.T, 2, par ^ 1);
-      if (kern == 3) hipLaunchKernelGGL(kP3<TM>, dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
+      if (kern == 3) launchP3<TM>(P, st, par);
       if (kern == 4) hipLaunchKernelGGL(kDecide<TM>, dim3(P.nSlots), dim3(kTD), 0, st, P.A, P.T, par);
       if (timed) DV_HIP(hipEventRecord(ev[2 * k + 1], st));
     }
@@ -1080,6 +1162,7 @@ Plan<TM> activePlan(Engine& E) {
   P.T = tablesOf<TM>(E, E.active);
   grids<TM>(E, E.active, P.gc, P.gb);
   P.nSlots = E.nSlots;
+  P.rpt = E.rowsPerThread;
   return P;
 }
 
@@ -1182,8 +1265,9 @@ void openT(Engine& E, int nSlots) {
 
 }  // namespace
 
-void engineOpen(Engine& E, int nSlots, bool mixed, int numCU, bool noGraph) {
+void engineOpen(Engine& E, int nSlots, bool mixed, int numCU, bool noGraph, int rowsPerThread) {
   if (nSlots < 1) throw DvError{DV_ERR_ARG, "an engine needs at least one slot"};
+  E.rowsPerThread = rowsPerThread == 2 ? 2 : 1;
   if (E.mixed != (mixed ? 1 : 0) || E.nSlots != nSlots) E.graph.reset();
   E.mixed = mixed ? 1 : 0;
   E.nSlots = nSlots;
@@ -1299,14 +1383,14 @@ void solveAll(Engine& E, const std::vector<Scene*>& scenes, hipStream_t st, int 
 void solveFusedMany(Engine& E, const std::vector<Scene*>& scenes, bool mixed, hipStream_t st, int numCU,
                     bool noGraph) {
   if (scenes.empty()) return;
-  engineOpen(E, (int)scenes.size(), mixed, numCU, noGraph);
+  engineOpen(E, (int)scenes.size(), mixed, numCU, noGraph, scenes[0]->rowsPerThread);
   if (mixed) solveAll<float>(E, scenes, st, 0, nullptr, nullptr);
   else solveAll<double>(E, scenes, st, 0, nullptr, nullptr);
 }
 
 void solveFused(Scene& s, bool mixed) {
   std::vector<Scene*> one{&s};
-  engineOpen(s.engine, 1, mixed, s.numCU, s.noGraph);
+  engineOpen(s.engine, 1, mixed, s.numCU, s.noGraph, s.rowsPerThread);
   if (mixed) solveAll<float>(s.engine, one, s.stream, s.profKernel, &s.profEv, &s.stats);
   else solveAll<double>(s.engine, one, s.stream, s.profKernel, &s.profEv, &s.stats);
 }

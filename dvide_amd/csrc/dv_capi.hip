@@ -333,6 +333,10 @@ int dv_set_param(dv_scene* h, int param, double v) {
         if (v < 0 || v > 4) return dv::fail(s, DV_ERR_ARG, "profile kernel is 0..4");
         s.profKernel = (int)v;
         break;
+      case DV_PARAM_ROWS_PER_THREAD:
+        if (v != 1 && v != 2) return dv::fail(s, DV_ERR_ARG, "rows per thread is 1 or 2");
+        s.rowsPerThread = (int)v;
+        break;
       default: return dv::fail(s, DV_ERR_ARG, "unknown parameter");
     }
     return DV_OK;
@@ -357,6 +361,7 @@ int dv_get_param(const dv_scene* hc, int param, double* v) {
       case DV_PARAM_VERBOSE: *v = s.verbose; break;
       case DV_PARAM_GRID: *v = s.grid; break;
       case DV_PARAM_PROFILE: *v = s.profKernel; break;
+      case DV_PARAM_ROWS_PER_THREAD: *v = s.rowsPerThread; break;
       default: return dv::fail(s, DV_ERR_ARG, "unknown parameter");
     }
     return DV_OK;
@@ -792,7 +797,7 @@ void serverOpen(dv_server& S, int solver, int maxSlots, double budget) {
   DV_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
   DV_HIP(hipEventCreateWithFlags(&S.solved, hipEventDisableTiming));
   S.slot.assign(maxSlots, dv_server::Request{-1, nullptr, 0});
-  engineOpen(S.engine, maxSlots, solver == DV_SOLVER_APGD_MIXED, S.numCU, false);
+  engineOpen(S.engine, maxSlots, solver == DV_SOLVER_APGD_MIXED, S.numCU, false, 1);
 }
 
 bool inFlight(const dv_server& S, const dv_scene* h) {

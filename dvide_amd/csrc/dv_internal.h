@@ -109,6 +109,7 @@ struct TileSet {
 // their setup) and leave (after their finalisation) between chunks.
 struct Engine {
   int mixed = -1;       // 1: FP32 products; 0: FP64
+  int rowsPerThread = 1;  // contact kernels: rows per thread (1 or 2)
   int nSlots = 0;
   std::vector<char> hostTable;  // Args<TM>[nSlots]
   std::vector<Scene*> slotScene;
@@ -138,6 +139,7 @@ struct Scene {
   int grid = 0;
   bool noGraph = false;  // launch the solver passes directly (profiling)
   int profKernel = 0;    // DV_PARAM_PROFILE
+  int rowsPerThread = 1; // DV_PARAM_ROWS_PER_THREAD
   std::vector<hipEvent_t> profEv;
   double time = 0;
   int timeIndex = 0;
@@ -224,7 +226,7 @@ void solveFusedMany(Engine& E, const std::vector<Scene*>& scenes, bool mixed, hi
 // continuous batching on an engine (dv_apgd_fused.hip): slots are admitted
 // (scene assembled, setup launched) and retired (gamma written back, stats
 // filled) between chunks of passes
-void engineOpen(Engine& E, int nSlots, bool mixed, int numCU, bool noGraph);
+void engineOpen(Engine& E, int nSlots, bool mixed, int numCU, bool noGraph, int rowsPerThread);
 void engineAdmit(Engine& E, int slot, Scene& s);
 void engineStartAdmitted(Engine& E, hipStream_t st);
 void engineRunChunk(Engine& E, hipStream_t st);

@@ -23,7 +23,7 @@ DV_OK = 0
 SOLVER_FUSED, SOLVER_MIXED, SOLVER_PLAIN = 1, 2, 3
 (PARAM_H, PARAM_MU, PARAM_TOL, PARAM_MAX_ITER, PARAM_GX, PARAM_GY, PARAM_GZ, PARAM_ENVELOPE,
  PARAM_WARM_START, PARAM_SOLVER, PARAM_GEOM_MODE, PARAM_TIME, PARAM_VERBOSE, PARAM_GRID,
- PARAM_PROFILE) = range(15)
+ PARAM_PROFILE, PARAM_ROWS_PER_THREAD) = range(16)
 PROFILE_OFF, PROFILE_P1, PROFILE_P2, PROFILE_P3, PROFILE_DECIDE = range(5)
 VEC_P, VEC_V, VEC_GAMMA, VEC_F_CONTACT, VEC_R, VEC_K, VEC_B = range(7)
 
@@ -279,6 +279,10 @@ class System:
 
     def setGrid(self, g):
         self._set(PARAM_GRID, g)
+
+    def setRowsPerThread(self, r):
+        """Contacts per thread of the solver's contact kernels (1 or 2)."""
+        self._set(PARAM_ROWS_PER_THREAD, r)
 
     def setProfileKernel(self, k):
         """Time one solver kernel per pass with HIP events (PROFILE_*)."""

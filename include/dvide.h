@@ -70,6 +70,9 @@ typedef struct dv_scene dv_scene;
                                  passes of each solve are then launched on the scene's
                                  stream with events around that kernel (the sample);
                                  the rest run in the captured graph */
+#define DV_PARAM_ROWS_PER_THREAD 15 /* contact kernels of the graph-driven solvers: 1 or 2
+                                       contacts per thread (a tuning choice; results agree
+                                       to rounding, each choice is deterministic) */
 
 /* state vectors (dv_get_vector / dv_set_vector) */
 #define DV_VEC_P 0         /* System::p_d        3*nb */
