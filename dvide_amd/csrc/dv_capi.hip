@@ -314,7 +314,7 @@ int dv_set_param(dv_scene* h, int param, double v) {
         break;
       case DV_PARAM_ENVELOPE:
         if (!(v >= 0)) return dv::fail(s, DV_ERR_ARG, "envelope must be >= 0");
-        s.envelope = v;
+        s.envelope = v;  // the hash cell edge 2 (r_max + envelope) follows it each step
         break;
       case DV_PARAM_WARM_START: s.warm = v != 0; break;
       case DV_PARAM_SOLVER:

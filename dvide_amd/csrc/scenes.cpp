@@ -228,6 +228,7 @@ Scene* latticeBox(uint64_t seed, int perSide) {
 //   lidMode 0: a free lid as heavy as the spheres together, pressing with its
 //     weight (models/test_compaction.cu:272-277), released at -0.1 m/s, with
 //     APGD warm starting as that driver sets it (test_compaction.cu:245-247);
+//   lidMode 2: no lid (the settled pile under gravity alone);
 //   lidMode 1: a 50 kg lid driven at -0.1 m/s through a bilateral DOF row on
 //     its y coordinate (System.cu:522-532), its x and z held by two more rows.
 //     Driving a rigid lid at a fixed speed into a jammed packing of rigid
@@ -250,6 +251,11 @@ Scene* compaction(uint64_t seed, int n, double W, int sweeps, int lidMode, doubl
   for (int i = 0; i < n; ++i) top = std::max(top, x[3 * i + 1] + r[i]);
   addContainer(*s, W, top + 0.3, W, 0.05);
   const double lt = 0.05;
+  if (lidMode == 2) {
+    for (int i = 0; i < n; ++i) s->addSphere(x[3 * i], x[3 * i + 1], x[3 * i + 2], r[i], 2500.0);
+    s->params[P_WARM] = 1;
+    return s;
+  }
   // lid bottom face at the highest sphere top: the top layer is in contact
   int lid = s->addBox(0.5 * W, top + lt, 0.5 * W, 0.5 * W - 0.001, lt, 0.5 * W - 0.001, false,
                       50.0);

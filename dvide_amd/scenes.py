@@ -111,7 +111,7 @@ def lattice_box(seed: int = 1001, per_side: int = 10) -> Scene:
     return generate(LATTICE_BOX, seed, per_side)
 
 
-LID_FREE, LID_BILATERAL = 0, 1
+LID_FREE, LID_BILATERAL, LID_NONE = 0, 1, 2
 
 
 def compaction(seed: int = 2002, n: int = 112000, width: float = 5.0, lid: int = LID_FREE,

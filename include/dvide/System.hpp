@@ -156,6 +156,10 @@ class System {
     }
     if (n) check(dv_add_bodies(s_, (int)n, pos.data(), vel.data(), mass.data(), geom.data(), fixed.data(),
                                family.data()));
+    // gravity enters the applied forces once, here (System.cu:344-424)
+    check(dv_set_param(s_, DV_PARAM_GX, gravity.x));
+    check(dv_set_param(s_, DV_PARAM_GY, gravity.y));
+    check(dv_set_param(s_, DV_PARAM_GZ, gravity.z));
     pushParams();
     check(dv_initialize(s_));
     return 0;
@@ -224,9 +228,6 @@ class System {
     if (apgd_.antiRelax) throw Error(DV_ERR_UNSUPPORTED, "anti-relaxation is not available on the GPU path");
     check(dv_set_param(s_, DV_PARAM_H, h));
     check(dv_set_param(s_, DV_PARAM_MU, mu));
-    check(dv_set_param(s_, DV_PARAM_GX, gravity.x));
-    check(dv_set_param(s_, DV_PARAM_GY, gravity.y));
-    check(dv_set_param(s_, DV_PARAM_GZ, gravity.z));
     check(dv_set_param(s_, DV_PARAM_TOL, solver->tolerance));
     check(dv_set_param(s_, DV_PARAM_MAX_ITER, solver->maxIterations));
     check(dv_set_param(s_, DV_PARAM_VERBOSE, solver->verbose ? 1 : 0));
