@@ -296,6 +296,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--settle", type=int, default=SETTLE_STEPS, help="untimed settling steps of the deposit")
+    ap.add_argument("--max-iterations", type=int, default=20000,
+                    help="APGD iteration cap of the timed steps (high enough for every step to reach the tolerance)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-mixed", action="store_true")
@@ -315,22 +317,28 @@ def main():
     from dvide_amd import scenes, system
 
     scene, settle_its = settled_scene(args.settle)
-    mixed = run_gpu(scene, system.SOLVER_MIXED, args.steps, args.warmup, profile_kernel=system.PROFILE_P1)
+    cap = args.max_iterations
+    mixed = run_gpu(scene, system.SOLVER_MIXED, args.steps, args.warmup, profile_kernel=system.PROFILE_P1,
+                    max_iterations=cap)
     others = {}
     if not args.only_mixed:
-        others["fp64"] = run_gpu(scene, system.SOLVER_FUSED, args.steps, args.warmup)
-        others["plain_fp64"] = run_gpu(scene, system.SOLVER_PLAIN, max(2, args.steps // 4), args.warmup and 1)
+        others["fp64"] = run_gpu(scene, system.SOLVER_FUSED, args.steps, args.warmup, max_iterations=cap)
+        others["plain_fp64"] = run_gpu(scene, system.SOLVER_PLAIN, max(2, args.steps // 4), args.warmup and 1,
+                                       max_iterations=cap)
 
     value = args.steps / mixed["elapsed"]
     nc = int(np.mean(mixed["contacts"]))
     k_us = 1e3 * mixed["kernel_ms"] / max(mixed["kernel_count"], 1)
     alg = p1_bytes_per_contact(True) * nc
     achieved = alg / (k_us * 1e-6) / 1e9 if k_us > 0 else 0.0
+    # HBM bytes per P1 launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # summary (scaled by a copy kernel); measured on the same workload
     traffic = None
-    tpath = os.path.join(_HERE, "profiles", "p1_traffic_compaction.json")
+    tpath = os.path.join(_HERE, "profiles", "apgd_traffic_compaction_112k_mixed.md")
     if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("mixed_bytes_per_launch")
+        for ln in open(tpath):
+            if ln.startswith("p1_traffic_bytes_per_launch:"):
+                traffic = int(float(ln.split(":")[1]))
 
     def lat_ms(r):
         return dict(p50=round(1e3 * float(np.percentile(r["lat"], 50)), 3),
@@ -358,7 +366,9 @@ def main():
         },
         "latency_ms": lat_ms(mixed),
         "iterations_mean": round(float(np.mean(mixed["iterations"])), 1),
+        "iterations_max": int(np.max(mixed["iterations"])),
         "residual_max": float(np.max(mixed["residual"])),
+        "steps_converged": int(np.sum(mixed["residual"] < TOL)),
         "phases_ms_last_step": {k: round(v, 3) for k, v in mixed["phases"].items()},
         "roofline": {
             "bound": "hbm",

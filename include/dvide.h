@@ -66,10 +66,11 @@ typedef struct dv_scene dv_scene;
 #define DV_PARAM_GRID 13      /* unused (accepted for compatibility) */
 #define DV_PARAM_PROFILE 14   /* time one kernel of the graph-driven solvers with HIP
                                  events: 0 off, 1 P1 (contacts), 2 P2 (bodies),
-                                 3 P3 (contacts), 4 the decide kernel.  The first 16
-                                 passes of each solve are then launched on the scene's
-                                 stream with events around that kernel (the sample);
-                                 the rest run in the captured graph */
+                                 3 P3 (contacts), 4 the decide kernel.  The first chunk
+                                 of 16 passes of each solve is then launched on the
+                                 scene's stream, and in its first pass that kernel runs
+                                 16 times back to back between two events (the decide
+                                 kernel once); the rest run in the captured graph */
 #define DV_PARAM_ROWS_PER_THREAD 15 /* contact kernels of the graph-driven solvers: 1 or 2
                                        contacts per thread (a tuning choice; results agree
                                        to rounding, each choice is deterministic) */
@@ -97,9 +98,8 @@ typedef struct dv_stats {
   double ms_solve;
   double ms_update;
   double time;            /* System::time after the step */
-  double prof_kernel_ms;  /* DV_PARAM_PROFILE: summed duration of the timed kernel
-                             over the sampled passes */
-  int prof_kernel_count;  /* ... and the number of those launches (passes that ran) */
+  double prof_kernel_ms;  /* DV_PARAM_PROFILE: summed duration of the timed launches */
+  int prof_kernel_count;  /* ... and their number */
   int passes;             /* solver passes (Schur products) of the step */
 } dv_stats;
 
