@@ -83,7 +83,7 @@ def run_gpu(scene, solver, steps, warmup, profile_kernel=0, max_iterations=None)
             g.step()
         g.setProfileKernel(profile_kernel)
         hip.hipDeviceSynchronize()
-        lat, its, res, nc, passes, kms, kcnt = [], [], [], [], [], 0.0, 0
+        lat, its, res, nc, passes, solve, kms, kcnt = [], [], [], [], [], [], 0.0, 0
         t0 = time.perf_counter()
         for _ in range(steps):
             ts = time.perf_counter()
@@ -94,6 +94,7 @@ def run_gpu(scene, solver, steps, warmup, profile_kernel=0, max_iterations=None)
             res.append(st.residual)
             nc.append(st.num_contacts)
             passes.append(st.passes)
+            solve.append(st.ms_solve)
             kms += st.prof_kernel_ms
             kcnt += st.prof_kernel_count
         hip.hipDeviceSynchronize()
@@ -101,7 +102,8 @@ def run_gpu(scene, solver, steps, warmup, profile_kernel=0, max_iterations=None)
         last = g.stats()
         return dict(
             elapsed=elapsed, lat=np.array(lat), iterations=np.array(its), residual=np.array(res),
-            contacts=np.array(nc), passes=np.array(passes), kernel_ms=kms, kernel_count=kcnt,
+            contacts=np.array(nc), passes=np.array(passes), solve_ms=np.array(solve), kernel_ms=kms,
+            kernel_count=kcnt,
             phases=dict(collision=last.ms_collision, assembly=last.ms_assembly, solve=last.ms_solve,
                         update=last.ms_update),
             nb=g.numBodies, v=g.v,
@@ -296,8 +298,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--settle", type=int, default=SETTLE_STEPS, help="untimed settling steps of the deposit")
-    ap.add_argument("--max-iterations", type=int, default=20000,
-                    help="APGD iteration cap of the timed steps (high enough for every step to reach the tolerance)")
+    ap.add_argument("--max-iterations", type=int, default=100000,
+                    help="APGD iteration cap of the timed steps (the reference's default, APGD.cu:11)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-mixed", action="store_true")
@@ -383,6 +385,10 @@ def main():
             "bytes_per_contact": p1_bytes_per_contact(True),
         },
     }
+    def us_per_iteration(r):
+        return round(1e3 * float(np.sum(r["solve_ms"])) / max(int(np.sum(r["iterations"])), 1), 2)
+
+    line["us_per_iteration"] = us_per_iteration(mixed)
     for name, r in others.items():
         k = len(r["lat"])
         line[name] = {
@@ -391,11 +397,24 @@ def main():
             "latency_ms": lat_ms(r),
             "iterations_mean": round(float(np.mean(r["iterations"])), 1),
             "residual_max": float(np.max(r["residual"])),
+            "steps_converged": int(np.sum(r["residual"] < TOL)),
+            "us_per_iteration": us_per_iteration(r),
         }
-    if "plain_fp64" in line:
-        line["mixed_vs_plain_fp64"] = round(value / line["plain_fp64"]["value"], 2)
-    if "fp64" in line:
-        line["mixed_vs_fp64"] = round(value / line["fp64"]["value"], 2)
+    # ratios over the same steps from the same state: the plain path runs the
+    # first steps of the window only
+    if "plain_fp64" in others:
+        k = len(others["plain_fp64"]["lat"])
+        line["mixed_vs_plain_fp64"] = {
+            "time_per_step": round(float(np.sum(others["plain_fp64"]["lat"]) / np.sum(mixed["lat"][:k])), 2),
+            "steps": k,
+            "time_per_iteration": round(line["plain_fp64"]["us_per_iteration"] / line["us_per_iteration"], 2),
+        }
+    if "fp64" in others:
+        line["mixed_vs_fp64"] = {
+            "time_per_step": round(float(np.sum(others["fp64"]["lat"]) / np.sum(mixed["lat"])), 2),
+            "steps": args.steps,
+            "time_per_iteration": round(line["fp64"]["us_per_iteration"] / line["us_per_iteration"], 2),
+        }
     line["settle_iterations"] = settle_its
     if not args.only_mixed:
         line["lid_pressed"] = lid_pressed()

@@ -137,19 +137,29 @@ void runSolver(Scene& s) {
 
 // First half of DoTimeStep (System.cu:544-568): collision detection and the
 // assembly of k, D and r on the scene's stream; returns whether there are
-// rows to solve.
-bool stepAssemble(Scene& s) {
+// rows to solve.  stepAssembleBegin only queues the collision counts, so that
+// a batch can queue those of all its scenes before waiting on any.
+void stepAssembleBegin(Scene& s) {
   if (!s.initialized) throw DvError{DV_ERR_STATE, "DoTimeStep before initializeSystem"};
-  const hipStream_t st = s.stream;
   s.stats = dv_stats{};
-  DV_HIP(hipEventRecord(s.ev[0], st));
-  collide(s);
+  DV_HIP(hipEventRecord(s.ev[0], s.stream));
+  collideCount(s);
+}
+
+bool stepAssembleEnd(Scene& s) {
+  const hipStream_t st = s.stream;
+  collideEmit(s);
   DV_HIP(hipEventRecord(s.ev[1], st));
   buildK(s);
   const bool rows = (s.nc + s.nbil) > 0;
   if (rows) buildR(s);
   DV_HIP(hipEventRecord(s.ev[2], st));
   return rows;
+}
+
+bool stepAssemble(Scene& s) {
+  stepAssembleBegin(s);
+  return stepAssembleEnd(s);
 }
 
 // Second half (System.cu:569-590): the update after the solve, time and
@@ -750,9 +760,10 @@ void batchStep(dv_batch& B) {
   B.stats = dv_batch_stats{};
   std::vector<Scene*> solve;
   std::vector<char> rows(B.scenes.size());
+  for (dv_scene* h : B.scenes) stepAssembleBegin(h->s);  // all counts queued first
   for (size_t i = 0; i < B.scenes.size(); ++i) {
     Scene& s = B.scenes[i]->s;
-    rows[i] = stepAssemble(s);
+    rows[i] = stepAssembleEnd(s);
     if (rows[i]) {
       DV_HIP(hipStreamWaitEvent(B.stream, s.ev[2], 0));
       solve.push_back(&s);

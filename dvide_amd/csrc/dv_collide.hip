@@ -373,8 +373,9 @@ void collideInit(Scene& s) {
   DV_HIP(hipStreamSynchronize(s.stream));
 }
 
-void collide(Scene& s) {
-  const hipStream_t st = s.stream;
+namespace {
+
+CdArgs cdArgs(Scene& s) {
   CdArgs a;
   a.p = s.p.p;
   a.geom = s.geom.p;
@@ -392,6 +393,14 @@ void collide(Scene& s) {
   a.mode = s.geomMode;
   a.slotStart = s.slotStart.p;
   a.cellItems = s.cellItems.p;
+  return a;
+}
+
+}  // namespace
+
+void collideCount(Scene& s) {
+  const hipStream_t st = s.stream;
+  const CdArgs a = cdArgs(s);
   const unsigned T = s.hashSize;
 
   hipLaunchKernelGGL(zeroInt, dim3(1), dim3(64), 0, st, s.counters.p, 8);
@@ -422,7 +431,15 @@ void collide(Scene& s) {
   exclusiveScan(s, tot.p, s.iStart.p, s.nb);
   DV_HIP(hipMemcpyAsync(s.pinned, s.cStart.p + s.nb, sizeof(int), hipMemcpyDeviceToHost, st));
   DV_HIP(hipMemcpyAsync(s.pinned + 1, s.iStart.p + s.nb, sizeof(int), hipMemcpyDeviceToHost, st));
-  DV_HIP(hipStreamSynchronize(st));
+}
+
+void collideEmit(Scene& s) {
+  const hipStream_t st = s.stream;
+  const CdArgs a = cdArgs(s);
+  const int nchunks = blocksFor(s.nb, kChunk);
+  int* chUp = s.bigChunks.p;
+  int* chDown = s.bigChunks.p + (size_t)std::max(1, s.nbig * nchunks);
+  DV_HIP(hipStreamSynchronize(st));  // the counts of collideCount
   const int nc = s.pinned[0];
   const int ninc = s.pinned[1];
   if (ninc != 2 * nc) throw DvError{DV_ERR_STATE, "collision detection: asymmetric pair counts"};
@@ -449,6 +466,11 @@ void collide(Scene& s) {
   hipLaunchKernelGGL(buildIncidence, dim3(blocksFor(s.nb, kT)), dim3(kT), 0, st, s.nb, s.cStart.p,
                      s.iStart.p, s.cntUp.p, s.cntDown.p, s.cB.p, s.inc.p, s.heavy.p, s.counters.p);
   DV_HIP(hipGetLastError());
+}
+
+void collide(Scene& s) {
+  collideCount(s);
+  collideEmit(s);
 }
 
 }  // namespace dv

@@ -205,6 +205,11 @@ struct Scene {
 // collision detection + incidence lists (dv_collide.hip)
 void collideInit(Scene& s);
 void collide(Scene& s);
+// the two halves of collide: counts (queued, no wait), then the wait for
+// them and the emission of the contacts; many scenes can queue their counts
+// before the first wait
+void collideCount(Scene& s);
+void collideEmit(Scene& s);
 
 // assembly of k and r, update (dv_assemble.hip)
 void buildK(Scene& s);
