@@ -1339,8 +1339,10 @@ bool engineBusy(const Engine& E) {
 namespace {
 
 // Admit all, run chunks until every scene is done, retire all.  With
-// profiling (single scene), the first chunk is launched on the stream with
-// events around the chosen kernel: the timed sample.
+// profiling (single scene), one chunk per solve is launched on the stream
+// with events around the chosen kernel: the first chunk after the first
+// whose entry state (the last state the host read back) has the momentum
+// term on, so that the timed contact pass moves all of its operands.
 template <typename TM>
 void solveAll(Engine& E, const std::vector<Scene*>& scenes, hipStream_t st, int prof,
               std::vector<hipEvent_t>* profEv, dv_stats* profStats) {
@@ -1361,8 +1363,10 @@ void solveAll(Engine& E, const std::vector<Scene*>& scenes, hipStream_t st, int 
   const long passCap = (long)maxIt * 64 + 4096;
   int passesBefore = 0;
   ApgdState* outH = reinterpret_cast<ApgdState*>(E.outHost);
+  bool sampled = false;
   for (;;) {
-    const bool timed = ev && E.launched == 0;
+    const bool timed = ev && !sampled && E.launched > 0 && !outH[0].restart && !outH[0].done;
+    sampled = sampled || timed;
     runChunk<TM>(E, st, timed ? ev : nullptr, prof);
     DV_HIP(hipMemcpyAsync(outH, E.out.p, sizeof(ApgdState) * E.nSlots, hipMemcpyDeviceToHost, st));
     DV_HIP(hipStreamSynchronize(st));
