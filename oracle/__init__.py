@@ -15,6 +15,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 P, V, GAMMA, F_CONTACT, R, B, K, FRICTION = range(8)
+# solvers, numbered as System(int solverType) numbers them (System.cu:77-101)
+APGD, PGJ, PGS = 1, 5, 6
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -36,6 +38,7 @@ def lib():
             "or_get_geom_mode": ([], ctypes.c_int),
             "or_set_threads": ([ctypes.c_int], ctypes.c_int),
             "or_set_params": ([vp, _dp], None),
+            "or_set_solver": ([vp, ctypes.c_int], ctypes.c_int),
             "or_set_bins": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], None),
             "or_add_bodies": ([vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _ip], ctypes.c_int),
             "or_add_bilateral": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double], ctypes.c_int),
@@ -79,7 +82,7 @@ class OracleSystem:
     """CPU restatement of System (System.cu) driven like the reference's drivers."""
 
     def __init__(self, scene, *, tol=None, max_iterations=None, h=None, mu=None,
-                 warm_start=None, anti_relax=False, bins=None, geom_mode=1):
+                 warm_start=None, anti_relax=False, bins=None, geom_mode=1, solver=APGD):
         L = lib()
         L.or_set_geom_mode(int(geom_mode))
         self._L = L
@@ -95,6 +98,8 @@ class OracleSystem:
                       prm["envelope"], prm["bins"], float(bool(prm["warm_start"])), float(bool(anti_relax))],
                      dtype=np.float64)
         L.or_set_params(self._h, _d(a))
+        if L.or_set_solver(self._h, int(solver)) != 0:
+            raise ValueError("solver is APGD, PGJ or PGS")
         pos = np.ascontiguousarray(scene.pos, np.float64)
         vel = np.ascontiguousarray(scene.vel, np.float64)
         mass = np.ascontiguousarray(scene.mass, np.float64)

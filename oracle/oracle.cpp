@@ -13,6 +13,7 @@
 //   buildContactJacobian (+ bilateral rows)        System.cu:622-638, 676-767, 904-967, 1321-1409
 //   buildSchurVector / buildStabilization          System.cu:1516-1544, 1561-1580
 //   APGD::solve, project, getResidual              APGD.cu:47-122, 164-361
+//   PGJ / PGS (secondary cross-check solvers)      PGJ.cu:66-81, 151-210; PGS.cu:149-252, 384-447
 //   DoTimeStep update                              System.cu:544-602
 //   getCCPViolation                                System.cu:1590-1643
 //
@@ -28,8 +29,9 @@
 // closed-form sphere/box geometry, the closed form of the Coulomb-cone
 // projection and a numerical nearest-point search, dense D M^-1 D^T products,
 // free fall by semi-implicit Euler, the ball-on-box normal impulse m g h and
-// the stick/slip threshold |g_x| = mu g, the pile weight balance, and the CCP
-// optimality conditions at convergence.
+// the stick/slip threshold |g_x| = mu g, the pile weight balance, the CCP
+// optimality conditions at convergence, and the velocities of converged
+// projected Gauss-Seidel and Jacobi runs (another route to the same CCP).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -609,6 +611,108 @@ struct System {
   }
   std::vector<double> antiRelaxVec;
 
+  // --- projected Jacobi and Gauss-Seidel (PGJ.cu:151-210, PGS.cu:149-233,
+  // 236-252, 384-447): the reference's secondary solvers of the same cone
+  // complementarity problem.  They reach the CCP solution by another route,
+  // so the velocities they give (unique, unlike gamma) cross-check APGD's.
+  // Like the reference they handle contact rows only (no bilateral rows).
+  int solverKind = 1;  // System(int solverType) numbering: 1 APGD, 5 PGJ, 6 PGS
+
+  // per-contact step scale B_c = 3 / ((m_A^-1 + m_B^-1) |[n; u; v]|^2)
+  // (PGS.cu:236-252, PGJ.cu:66-81), from the body-A block of D
+  void pgScale(std::vector<double>& B) const {
+    B.assign(3 * (size_t)nc, 0.0);
+    for (int c = 0; c < nc; ++c) {
+      const double* d = &D.V[18 * (size_t)c];
+      const double mA = minv[3 * (size_t)idA[c]], mB = minv[3 * (size_t)idB[c]];
+      double sq = 0;
+      const int at[9] = {0, 1, 2, 6, 7, 8, 12, 13, 14};
+      for (int q = 0; q < 9; ++q) sq += d[at[q]] * d[at[q]];
+      const double g = 3.0 / ((mA + mB) * sq);
+      B[3 * c] = B[3 * c + 1] = B[3 * c + 2] = g;
+    }
+  }
+
+  // stopping measure of both (PGS.cu:423-441): the largest of the cone
+  // infeasibility of gamma, the dual-cone infeasibility of w = N gamma + r
+  // and the per-contact complementarity |gamma_c . w_c|
+  double pgResidual(std::vector<double>& w) {
+    schur(gamma, w);
+    for (size_t i = 0; i < w.size(); ++i) w[i] = 1.0 * r[i] + w[i];
+    double fx = 0, fy = 0, comp = 0;
+    for (int c = 0; c < nc; ++c) {
+      const double m = friction[c];
+      const double* g = &gamma[3 * (size_t)c];
+      const double* x = &w[3 * (size_t)c];
+      double a = m * g[0] - std::sqrt(g[1] * g[1] + g[2] * g[2]);
+      if (a != a) a = 0.0;
+      fx = std::max(fx, -std::min(0.0, a));
+      double bb = (1.0 / m) * x[0] - std::sqrt(x[1] * x[1] + x[2] * x[2]);
+      if (bb != bb) bb = 0.0;
+      fy = std::max(fy, -std::min(0.0, bb));
+      comp = std::max(comp, std::fabs(x[0] * g[0] + x[1] * g[1] + x[2] * g[2]));
+    }
+    return std::max(std::max(fx, fy), comp);
+  }
+
+  void projectedIterations() {
+    const double omega = 0.3, lambda = 2.0 / 3.0;  // PGS.cu:13-14, PGJ.cu:14-15
+    iterations = -1;
+    if (!bilA.empty()) return;  // contact rows only
+    gamma.assign(3 * (size_t)nc, 0.0);
+    for (int c = 0; c < nc; ++c) gamma[3 * c] = 1.0;
+    std::vector<double> B, w, gh;
+    pgScale(B);
+    std::vector<double> vel(3 * (size_t)nb);
+    if (solverKind == 6) {  // v = M^-1 (D^T gamma) + M^-1 k
+      DT.mul(gamma, fContact);
+      M.mul(fContact, vel);
+      M.mul(k, tmp);
+      for (size_t i = 0; i < vel.size(); ++i) vel[i] = 1.0 * tmp[i] + vel[i];
+    }
+    double res = 0;
+    int it;
+    for (it = 0; it < maxIt; ++it) {
+      if (solverKind == 6) {
+        // one Gauss-Seidel sweep in contact order, velocities updated in place
+        for (int c = 0; c < nc; ++c) {
+          const double* d = &D.V[18 * (size_t)c];
+          const V3 n = mk(d[0], d[1], d[2]), u = mk(d[6], d[7], d[8]), t = mk(d[12], d[13], d[14]);
+          const size_t a = 3 * (size_t)idA[c], bq = 3 * (size_t)idB[c];
+          const V3 vA = mk(vel[a], vel[a + 1], vel[a + 2]), vB = mk(vel[bq], vel[bq + 1], vel[bq + 2]);
+          double* g = &gamma[3 * (size_t)c];
+          const double old[3] = {g[0], g[1], g[2]};
+          double x[3];
+          x[0] = old[0] - omega * B[3 * c] * (dot(n, vA) - dot(n, vB) + b[3 * c]);
+          x[1] = old[1] - omega * B[3 * c + 1] * (dot(u, vA) - dot(u, vB));
+          x[2] = old[2] - omega * B[3 * c + 2] * (dot(t, vA) - dot(t, vB));
+          projectOne(x, friction[c]);
+          double dl[3];
+          for (int q = 0; q < 3; ++q) {
+            g[q] = lambda * x[q] + (1.0 - lambda) * old[q];
+            dl[q] = lambda * (x[q] - old[q]);
+          }
+          const V3 imp = mk(n.x * dl[0] + u.x * dl[1] + t.x * dl[2], n.y * dl[0] + u.y * dl[1] + t.y * dl[2],
+                            n.z * dl[0] + u.z * dl[1] + t.z * dl[2]);
+          const double mA = minv[a], mB = minv[bq];
+          vel[a] = vA.x + mA * imp.x; vel[a + 1] = vA.y + mA * imp.y; vel[a + 2] = vA.z + mA * imp.z;
+          vel[bq] = vB.x - mB * imp.x; vel[bq + 1] = vB.y - mB * imp.y; vel[bq + 2] = vB.z - mB * imp.z;
+        }
+      } else {
+        // one Jacobi step: gamma := lambda Pi(gamma - omega B (N gamma + r)) + (1 - lambda) gamma
+        schur(gamma, w);
+        gh.resize(gamma.size());
+        for (size_t i = 0; i < gamma.size(); ++i) gh[i] = gamma[i] - omega * (B[i] * (w[i] + r[i]));
+        project(gh);
+        for (size_t i = 0; i < gamma.size(); ++i) gamma[i] = lambda * gh[i] + (1.0 - lambda) * gamma[i];
+      }
+      res = pgResidual(w);
+      if (res < tol) break;
+    }
+    iterations = it;
+    residual = res;
+  }
+
   // --- System::DoTimeStep (System.cu:544-602) --------------------------------
   void step() {
     collide();
@@ -618,7 +722,8 @@ struct System {
       buildD();
       buildR();
       antiRelaxVec.assign(rows(), 0.0);
-      apgd();
+      if (solverKind == 5 || solverKind == 6) projectedIterations();
+      else apgd();
       DT.mul(gamma, fContact);
       for (int i = 0; i < 3 * nb; ++i) tmp[i] = 1.0 * k[i] + 1.0 * fContact[i];
       M.mul(tmp, v);
@@ -692,6 +797,13 @@ void or_set_params(void* sp, const double* a) {
   s->bins[0] = s->bins[1] = s->bins[2] = (unsigned)a[8];
   s->warm = a[9] != 0;
   s->antiRelax = a[10] != 0;
+}
+
+// solver of the step: 1 APGD (default), 5 PGJ, 6 PGS (System.cu:77-101)
+int or_set_solver(void* sp, int kind) {
+  if (kind != 1 && kind != 5 && kind != 6) return -1;
+  static_cast<System*>(sp)->solverKind = kind;
+  return 0;
 }
 
 void or_set_bins(void* sp, int bx, int by, int bz) {

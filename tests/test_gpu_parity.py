@@ -189,3 +189,41 @@ def test_free_fall_gpu(gpu):
     v1 = v0.reshape(-1, 3) + h * np.array(sc.params["gravity"])
     assert np.allclose(g.v.reshape(-1, 3), v1, rtol=0, atol=1e-15)
     assert np.allclose(g.p, p0 + h * g.v, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("solver,bound", [(system.SOLVER_FUSED, 1e-10), (system.SOLVER_MIXED, 1e-5)])
+def test_rows_per_thread_agree(gpu, solver, bound):
+    """One or two contacts per thread in the contact pass P1 only change the
+    order of the partial sums: the same ten iterations agree to rounding."""
+    sc = scenes.pile(31, 6000)
+    out = []
+    for r in (1, 2):
+        g = system.from_scene(sc, solver, tol=0.0, max_iterations=10)
+        g.setRowsPerThread(r)
+        g.step()
+        out.append((g.stats().iterations, g.gamma.copy(), g.v.copy()))
+        g.close()
+    assert out[0][0] == out[1][0] == 10
+    assert rel(out[1][1], out[0][1]) < bound
+    assert rel(out[1][2], out[0][2]) < bound
+
+
+def test_fp64_solver_velocities_match_pgs(gpu):
+    """The graph-driven FP64 APGD against the oracle's projected Gauss-Seidel
+    (PGS.cu:149-447), another algorithm for the same cone complementarity
+    problem: the velocities of the solution are unique, so they agree."""
+    sc = scenes.pile(5, 60)
+    prev = oracle.set_threads(0)
+    oracle.set_threads(1)
+    try:
+        p = oracle.OracleSystem(sc, tol=1e-10, max_iterations=4000000, solver=oracle.PGS)
+        p.step()
+    finally:
+        oracle.set_threads(prev)
+    assert p.residual < 1e-10
+    g = system.from_scene(sc, system.SOLVER_FUSED, tol=1e-8, max_iterations=100000)
+    g.step()
+    assert g.stats().residual < 1e-8
+    free = np.repeat(sc.fixed == 0, 3)
+    vp = p.vec(oracle.V)[free]
+    assert np.abs(g.v[free] - vp).max() < 1e-5 * np.abs(vp).max()

@@ -354,3 +354,39 @@ def test_residual_zero_at_known_solution():
     o.assemble()
     assert o.residual_of(np.array([9.81e-2, 0, 0])) < 1e-9
     assert o.residual_of(np.array([0.2, 0, 0])) > 1e-3
+
+
+@pytest.fixture
+def one_thread():
+    prev = oracle.set_threads(0)
+    oracle.set_threads(1)
+    yield
+    oracle.set_threads(prev)
+
+
+@pytest.mark.parametrize("kind", [oracle.PGS, oracle.PGJ])
+def test_projected_solvers_reach_the_apgd_velocities(one_thread, kind):
+    """PGS and PGJ (PGS.cu:149-447, PGJ.cu:151-210) solve the same cone
+    complementarity problem by another route.  The velocities of its solution
+    are unique (gamma is not), so converged runs of the two methods must give
+    the same v: an independent check of APGD's solution."""
+    sc = scenes.pile(5, 60)
+    free = np.repeat(sc.fixed == 0, 3)
+    a = oracle.OracleSystem(sc, tol=1e-8, max_iterations=100000)
+    p = oracle.OracleSystem(sc, tol=1e-10, max_iterations=4000000, solver=kind)
+    a.step()
+    p.step()
+    assert a.num_contacts == p.num_contacts > 20
+    assert a.residual < 1e-8
+    assert p.residual < 1e-10 and p.iterations < 4000000
+    va, vp = a.vec(oracle.V)[free], p.vec(oracle.V)[free]
+    assert np.abs(vp - va).max() < 1e-5 * np.abs(va).max()
+
+
+def test_projected_solvers_refuse_bilateral_rows(one_thread):
+    sc = scenes.compaction(2002, n=300, width=0.6, lid=scenes.LID_BILATERAL)
+    p = oracle.OracleSystem(sc, tol=1e-6, max_iterations=10, solver=oracle.PGS)
+    p.step()
+    assert p.iterations == -1
+    with pytest.raises(ValueError):
+        oracle.OracleSystem(sc, solver=2)
