@@ -70,6 +70,24 @@ struct ApgdState {
   int hatSrc, done, converged, passes;
 };
 
+// What the pass kernels read of a slot's state, written beside the state by
+// every decision.  The ring slots, the done and restart flags come first, in
+// one 16-byte load; the ring pointers follow by arithmetic from the slot's
+// ring base, so a pass kernel's data loads wait on a single round trip.
+struct alignas(32) PassView {
+  int cur, prev, nxt, flags;  // flags: 1 done, 2 restart
+  double t, beta;
+};
+constexpr int kViewDone = 1, kViewRestart = 2;
+
+__device__ __forceinline__ PassView viewOf(const ApgdState& S) {
+  PassView v;
+  v.cur = S.cur; v.prev = S.prev; v.nxt = S.nxt;
+  v.flags = (S.done ? kViewDone : 0) | (S.restart ? kViewRestart : 0);
+  v.t = S.t; v.beta = S.beta;
+  return v;
+}
+
 struct SolveDims {
   int nc, nbil, nb, maxIt;
   double mu, tol, gdiff;
@@ -86,9 +104,36 @@ constexpr int kPassesPerChunk = 16;  // even: a chunk ends on parity 0
 static_assert(kPassesPerChunk % 2 == 0, "pass parity must return to 0 at a chunk end");
 
 template <typename TM>
-struct V4 {
+struct alignas(4 * sizeof(TM)) V4 {
   TM x, y, z, w;
 };
+
+// Device code reaches the solver's arrays through pointers held in the scene
+// table, which the compiler can only treat as generic (flat) pointers.  Flat
+// loads also count against the LDS/scalar counter, so every wait for a
+// scalar load of the table would also wait for all outstanding data loads.
+// Casting to the global address space makes them global loads and stores.
+#define DV_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const DV_GLOBAL T* gl(const T* p) {
+  return (const DV_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ DV_GLOBAL T* gl(T* p) {
+  return (DV_GLOBAL T*)p;
+}
+template <typename TM>
+__device__ __forceinline__ V4<TM> ld4(const V4<TM>* p, long i) {
+  const DV_GLOBAL V4<TM>* q = gl(p) + i;
+  V4<TM> v;
+  v.x = q->x; v.y = q->y; v.z = q->z; v.w = q->w;
+  return v;
+}
+template <typename TM>
+__device__ __forceinline__ void st4(V4<TM>* p, long i, const V4<TM>& v) {
+  DV_GLOBAL V4<TM>* q = gl(p) + i;
+  q->x = v.x; q->y = v.y; q->z = v.z; q->w = v.w;
+}
 
 // Per-scene view of the solver, one entry of the device-resident scene table.
 template <typename TM>
@@ -116,6 +161,7 @@ struct Args {
   double* gammaIO;      // flat gamma: previous values in, result out
   double* ring[4];      // iterates (component-major), slots named by ApgdState
   TM* rn[4];            // N of the iterate in the same ring slot
+  long ringStride;      // ring[k] = ring[0] + k * ringStride, and the same for rn
   V4<TM>* J;
   V4<TM>* tmp;
   double* gHat;         // gammaHat before any iterate is accepted (ones)
@@ -133,6 +179,7 @@ struct Tables {
   const int* counts;    // [0] row tiles, [1] body tiles, [2] heavy bodies
   const int* slots;     // the set's slots (one decide workgroup each)
   ApgdState* st;        // [slot][2]: the solver state by pass parity
+  PassView* pv;         // [slot][2]: its view for the pass kernels
   ApgdState* out;       // per slot: the state after the latest decision
   int single;           // one slot: tile k is rows k*kTC.. of slot 0 (no table read)
 };
@@ -140,6 +187,17 @@ struct Tables {
 // state of slot `slot` at parity k: at a fixed offset from the kernel
 // argument, so reading it does not wait for a table load
 #define STATE(T, slot, k) ((T).st[2 * (slot) + (k)])
+#define VIEW(T, slot, k) ((T).pv[2 * (slot) + (k)])
+
+// A slot's pass view.  viewDone tests the ring slots (never negative) with
+// the done flag, so that the compiler reads the view's first 16 bytes in one
+// load before the branch instead of the flag alone and the slots after it.
+__device__ __forceinline__ PassView loadView(const Tables& T, int slot, int k) {
+  return VIEW(T, slot, k);
+}
+__device__ __forceinline__ bool viewDone(const PassView& S) {
+  return (S.flags & kViewDone) || (S.cur | S.prev | S.nxt) < 0;
+}
 
 // row offsets: contact c component k -> k*cap + c; bilateral b -> 3*cap + b
 #define ROW(a, k, c) ((long)(k) * (a).cap + (c))
@@ -183,7 +241,7 @@ template <int NT>
 __device__ void writePartials(double (&acc)[kSlots], double* part) {
   __shared__ double tot[kSlots];
   blockReduce<NT, kSlots>(acc, tot);
-  if (threadIdx.x < kSlots) part[threadIdx.x] = tot[threadIdx.x];
+  if (threadIdx.x < kSlots) gl(part)[threadIdx.x] = tot[threadIdx.x];
 }
 
 // Fixed-order totals of the per-tile partials of one scene by one wavefront
@@ -203,8 +261,8 @@ __device__ void reduceAll(const double* part1, const double* part3, int ng, doub
       const int ws = w < ng ? w : 0;
 #pragma unroll
       for (int s = 0; s < kSlots; ++s) {
-        x[k][s] = part1[(long)ws * kSlots + s];
-        x[k][kSlots + s] = part3[(long)ws * kSlots + s];
+        x[k][s] = gl(part1)[(long)ws * kSlots + s];
+        x[k][kSlots + s] = gl(part3)[(long)ws * kSlots + s];
       }
     }
 #pragma unroll
@@ -226,7 +284,7 @@ __device__ void reduceAll(const double* part1, const double* part3, int ng, doub
 
 __device__ __forceinline__ void ldFrame(const Args<double>& a, int c, int mode, double (&F)[9]) {
   const long S = a.cap;
-  const d3 n = mk3(a.frame[c], a.frame[S + c], a.frame[2 * S + c]);
+  const d3 n = mk3(gl(a.frame)[c], gl(a.frame)[S + c], gl(a.frame)[2 * S + c]);
   d3 u, v;
   contactFrame(n, mode, &u, &v);  // bit-identical to the assembled frame
   F[0] = n.x; F[1] = n.y; F[2] = n.z;
@@ -237,7 +295,7 @@ __device__ __forceinline__ void ldFrame(const Args<double>& a, int c, int mode, 
 __device__ __forceinline__ void ldFrame(const Args<float>& a, int c, int, float (&F)[9]) {
   const long S = a.cap;
 #pragma unroll
-  for (int k = 0; k < 6; ++k) F[k] = a.frame[k * S + c];
+  for (int k = 0; k < 6; ++k) F[k] = gl(a.frame)[k * S + c];
   F[6] = F[1] * F[5] - F[2] * F[4];  // v = n x u
   F[7] = F[2] * F[3] - F[0] * F[5];
   F[8] = F[0] * F[4] - F[1] * F[3];
@@ -250,26 +308,26 @@ __device__ __forceinline__ void storeJ(const Args<TM>& a, int c, const TM (&F)[9
   j.y = F[1] * x0 + F[4] * x1 + F[7] * x2;
   j.z = F[2] * x0 + F[5] * x1 + F[8] * x2;
   j.w = 0;
-  a.J[c] = j;
+  st4(a.J, c, j);
 }
 
 template <typename TM>
 __device__ __forceinline__ TM tmpDof(const Args<TM>& a, int dof) {
-  const V4<TM> t = a.tmp[dof / 3];
+  const V4<TM> t = ld4(a.tmp, dof / 3);
   const int k = dof % 3;
   return k == 0 ? t.x : (k == 1 ? t.y : t.z);
 }
 
 template <typename TM>
 __device__ __forceinline__ TM bilRow(const Args<TM>& a, int q) {
-  TM s = tmpDof(a, a.bilA[q]);
-  if (a.bilB[q] >= 0) s -= tmpDof(a, a.bilB[q]);
+  TM s = tmpDof(a, gl(a.bilA)[q]);
+  if (gl(a.bilB)[q] >= 0) s -= tmpDof(a, gl(a.bilB)[q]);
   return s;
 }
 
 template <typename TM>
 __device__ __forceinline__ void contactRows(const Args<TM>& a, int c, const TM (&F)[9], TM& n0, TM& n1, TM& n2) {
-  const V4<TM> tA = a.tmp[a.cA[c]], tB = a.tmp[a.cB[c]];
+  const V4<TM> tA = ld4(a.tmp, gl(a.cA)[c]), tB = ld4(a.tmp, gl(a.cB)[c]);
   const TM d0 = tA.x - tB.x, d1 = tA.y - tB.y, d2 = tA.z - tB.z;
   n0 = F[0] * d0 + F[1] * d1 + F[2] * d2;
   n1 = F[3] * d0 + F[4] * d1 + F[5] * d2;
@@ -326,20 +384,20 @@ __global__ __launch_bounds__(kTC) void kSetup1(const Args<TM>* __restrict__ A, T
       for (int rr = 0; rr < nr; ++rr) {
         const long f = bil ? (q - D.nc) : (D.nbil + 3L * q + rr);  // flat (reference) row
         const long o = bil ? BROW(a, q - D.nc) : ROW(a, rr, q);
-        a.r[o] = (TM)a.rFlat[f];
+        gl(a.r)[o] = (TM)gl(a.rFlat)[f];
         // flat entries 0..3nc-1 start as (1,0,0) triples, without the
         // bilateral offset (APGD.cu:116-122); the rest keep their old values
-        const double gv = (!D.warm && f < n3) ? ((f % 3 == 0) ? 1.0 : 0.0) : a.gammaIO[f];
-        a.ring[0][o] = gv;
-        a.gHat[o] = 1.0;
+        const double gv = (!D.warm && f < n3) ? ((f % 3 == 0) ? 1.0 : 0.0) : gl(a.gammaIO)[f];
+        gl(a.ring[0])[o] = gv;
+        gl(a.gHat)[o] = 1.0;
         x[rr] = 1.0 * gv + (-1.0) * 1.0;
-        a.ring[1][o] = x[rr];
+        gl(a.ring[1])[o] = x[rr];
         acc[0] += x[rr] * x[rr];
       }
       if (!bil) {
         if (kMixed) {
 #pragma unroll
-          for (int k = 0; k < 6; ++k) a.frame[k * a.cap + q] = (TM)a.frame64[k * a.cap + q];
+          for (int k = 0; k < 6; ++k) gl(a.frame)[k * a.cap + q] = (TM)gl(a.frame64)[k * a.cap + q];
         }
         TM F[9];
         ldFrame(a, (int)q, D.geomMode, F);
@@ -355,72 +413,71 @@ __global__ __launch_bounds__(kTC) void kSetup1(const Args<TM>* __restrict__ A, T
 // (gamma_0), 2 = ring[nxt] (gamma_new of the pass whose state is st[sIdx]),
 // 3 = gHat
 template <typename TM>
-__device__ __forceinline__ const double* bodySource(const Args<TM>& a, int which, const ApgdState& S) {
-  return which == 0 ? a.ring[1] : which == 1 ? a.ring[0] : which == 2 ? a.ring[S.nxt] : a.gHat;
+__device__ __forceinline__ const DV_GLOBAL double* bodySource(const Args<TM>& a, int which, const PassView& S) {
+  return gl(which == 0 ? a.ring[1] : which == 1 ? a.ring[0] : which == 2 ? a.ring[0] + S.nxt * a.ringStride : a.gHat);
 }
 
 template <typename TM>
 __device__ __forceinline__ TM bodyMinv(const Args<TM>& a, int g, int which) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   if (kMixed && which == 0) {
-    const TM mi = (TM)a.minv64[g];
-    a.minv[g] = mi;
+    const TM mi = (TM)gl(a.minv64)[g];
+    gl(a.minv)[g] = mi;
     return mi;
   }
-  return a.minv[g];
+  return gl(a.minv)[g];
 }
 
 template <typename TM>
 __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tables T, int which, int sIdx) {
   for (int ti = blockIdx.x, nt = T.counts[1]; ti < nt; ti += gridDim.x) {
     const Tile tile = bodyTile(T, ti);
-    const ApgdState& S = STATE(T, tile.scene, sIdx);
-    if (which == 2 && S.done) continue;
+    const PassView S = loadView(T, tile.scene, sIdx);
+    if (which == 2 && viewDone(S)) continue;
     const Args<TM>& a = A[tile.scene];
     const int g = tile.start + threadIdx.x;
     if (g >= a.dims.nb) continue;
-    const double* X = bodySource(a, which, S);
+    const DV_GLOBAL double* X = bodySource(a, which, S);
     const long bil0 = 3L * a.cap;
-    const int k0 = a.iStart[g], k1 = a.iStart[g + 1];
+    const int k0 = gl(a.iStart)[g], k1 = gl(a.iStart)[g + 1];
     if (k1 - k0 > kHeavyDegree) continue;  // reduced below by a workgroup
     const TM mi = bodyMinv(a, g, which);
     V4<TM> out;
     out.x = 0; out.y = 0; out.z = 0; out.w = 0;
     if (mi != (TM)0) {
       TM s0 = 0, s1 = 0, s2 = 0;
-      for (int k = a.bilIncStart[g]; k < a.bilIncStart[g + 1]; ++k) {
-        const int e = a.bilInc[k];
+      for (int k = gl(a.bilIncStart)[g]; k < gl(a.bilIncStart)[g + 1]; ++k) {
+        const int e = gl(a.bilInc)[k];
         const TM t = (TM)((e & 1) ? -X[bil0 + (e >> 3)] : X[bil0 + (e >> 3)]);
         const int comp = (e >> 1) & 3;
         if (comp == 0) s0 += t; else if (comp == 1) s1 += t; else s2 += t;
       }
       // incidences in batches of 8: all index loads, then all J loads, then
       // the sums (in incidence order), so the gathers overlap
+      // (slots past the body's last incidence repeat its first one, so that
+      // every load is unconditional)
       for (int kb = k0; kb < k1; kb += 8) {
         int e[8];
         V4<TM> j[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) e[u] = (kb + u < k1) ? a.inc[kb + u] : -1;
+        for (int u = 0; u < 8; ++u) e[u] = gl(a.inc)[kb + u < k1 ? kb + u : kb];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) j[u] = ld4(a.J, e[u] >> 1);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          if (e[u] >= 0) j[u] = a.J[e[u] >> 1];
-          else { j[u].x = 0; j[u].y = 0; j[u].z = 0; j[u].w = 0; }
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          if (e[u] < 0) continue;
+          if (kb + u >= k1) continue;
           if (e[u] & 1) { s0 -= j[u].x; s1 -= j[u].y; s2 -= j[u].z; }
           else { s0 += j[u].x; s1 += j[u].y; s2 += j[u].z; }
         }
       }
       out.x = mi * s0; out.y = mi * s1; out.z = mi * s2;
     }
-    a.tmp[g] = out;
+    st4(a.tmp, g, out);
   }
   for (int hq = blockIdx.x, nh = T.counts[2]; hq < nh; hq += gridDim.x) {
     const int2 ref = T.heavy[hq];
-    const ApgdState& S = STATE(T, ref.x, sIdx);
-    if (which == 2 && S.done) continue;
+    const PassView S = loadView(T, ref.x, sIdx);
+    if (which == 2 && viewDone(S)) continue;
     const Args<TM>& a = A[ref.x];
     const int g = ref.y;
     const TM mi = bodyMinv(a, g, which);
@@ -428,26 +485,26 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
       if (threadIdx.x == 0) {
         V4<TM> out;
         out.x = 0; out.y = 0; out.z = 0; out.w = 0;
-        a.tmp[g] = out;
+        st4(a.tmp, g, out);
       }
       continue;
     }
-    const double* X = bodySource(a, which, S);
+    const DV_GLOBAL double* X = bodySource(a, which, S);
     const long bil0 = 3L * a.cap;
-    const int k0 = a.iStart[g], k1 = a.iStart[g + 1];
+    const int k0 = gl(a.iStart)[g], k1 = gl(a.iStart)[g + 1];
     double acc[kSlots];
     zero(acc);
     for (int k = k0 + threadIdx.x; k < k1; k += kTB) {
-      const int e = a.inc[k];
-      const V4<TM> j = a.J[e >> 1];
+      const int e = gl(a.inc)[k];
+      const V4<TM> j = ld4(a.J, e >> 1);
       const double sg = (e & 1) ? -1.0 : 1.0;
       acc[0] += sg * (double)j.x;
       acc[1] += sg * (double)j.y;
       acc[2] += sg * (double)j.z;
     }
     if (threadIdx.x == 0) {
-      for (int k = a.bilIncStart[g]; k < a.bilIncStart[g + 1]; ++k) {
-        const int e = a.bilInc[k];
+      for (int k = gl(a.bilIncStart)[g]; k < gl(a.bilIncStart)[g + 1]; ++k) {
+        const int e = gl(a.bilInc)[k];
         acc[(e >> 1) & 3] += (e & 1) ? -X[bil0 + (e >> 3)] : X[bil0 + (e >> 3)];
       }
     }
@@ -459,7 +516,7 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
       out.y = mi * (TM)tot[1];
       out.z = mi * (TM)tot[2];
       out.w = 0;
-      a.tmp[g] = out;
+      st4(a.tmp, g, out);
     }
   }
 }
@@ -486,7 +543,7 @@ __global__ __launch_bounds__(kTC) void kSetup3(const Args<TM>* __restrict__ A, T
         TM n0, n1, n2;
         contactRows(a, c, F, n0, n1, n2);
         acc[1] += (double)n0 * (double)n0 + (double)n1 * (double)n1 + (double)n2 * (double)n2;
-        storeJ(a, c, F, (TM)a.ring[0][ROW(a, 0, c)], (TM)a.ring[0][ROW(a, 1, c)], (TM)a.ring[0][ROW(a, 2, c)]);
+        storeJ(a, c, F, (TM)gl(a.ring[0])[ROW(a, 0, c)], (TM)gl(a.ring[0])[ROW(a, 1, c)], (TM)gl(a.ring[0])[ROW(a, 2, c)]);
       }
     }
     writePartials<kTC>(acc, TILE_PART(a.part3[0], tile));
@@ -504,14 +561,14 @@ __global__ __launch_bounds__(kTC) void kSetup5(const Args<TM>* __restrict__ A, T
     const long q = (long)tile.start + threadIdx.x;
     if (q >= items) continue;
     if (q >= D.nc) {
-      a.rn[0][BROW(a, q - D.nc)] = bilRow(a, (int)(q - D.nc));
+      gl(a.rn[0])[BROW(a, q - D.nc)] = bilRow(a, (int)(q - D.nc));
     } else {
       const int c = (int)q;
       TM F[9];
       ldFrame(a, c, D.geomMode, F);
       TM n0, n1, n2;
       contactRows(a, c, F, n0, n1, n2);
-      a.rn[0][ROW(a, 0, c)] = n0; a.rn[0][ROW(a, 1, c)] = n1; a.rn[0][ROW(a, 2, c)] = n2;
+      gl(a.rn[0])[ROW(a, 0, c)] = n0; gl(a.rn[0])[ROW(a, 1, c)] = n1; gl(a.rn[0])[ROW(a, 2, c)] = n2;
     }
   }
 }
@@ -542,6 +599,8 @@ __global__ __launch_bounds__(kTD) void kSetupDecide(const Args<TM>* __restrict__
     S.passes = 0;
     STATE(T, slot, 0) = S;
     STATE(T, slot, 1) = S;
+    VIEW(T, slot, 0) = viewOf(S);
+    VIEW(T, slot, 1) = viewOf(S);
     T.out[slot] = S;
   }
 }
@@ -639,20 +698,23 @@ __global__ __launch_bounds__(kTC / R) void kP1(const Args<TM>* __restrict__ A, T
   constexpr int NF = kMixed ? 6 : 3;  // stored frame components
   TILE_LOOP(T) {
     const Tile tile = rowTile(T, ti_);
-    const ApgdState S = STATE(T, tile.scene, par ^ 1);
-    if (S.done) continue;
+    const PassView S = loadView(T, tile.scene, par ^ 1);
+    if (viewDone(S)) continue;
     const Args<TM>& a = A[tile.scene];
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
-    const double* Gk = a.ring[S.cur];
-    const double* Gkm1 = a.ring[S.prev];
-    const TM* Nk = a.rn[S.cur];
-    const TM* Nkm1 = a.rn[S.prev];
-    double* Gn = a.ring[S.nxt];
-    const bool mom = !S.restart;  // momentum term present
-    // loads of all rows of the thread
-    double gk[R][3], gm[R][3], rv[R][3];
-    TM nk[R][3], nm[R][3], fc[R][6];
+    const DV_GLOBAL double* Gk = gl(a.ring[0] + S.cur * a.ringStride);
+    const DV_GLOBAL double* Gkm1 = gl(a.ring[0] + S.prev * a.ringStride);
+    const DV_GLOBAL TM* Nk = gl(a.rn[0] + S.cur * a.ringStride);
+    const DV_GLOBAL TM* Nkm1 = gl(a.rn[0] + S.prev * a.ringStride);
+    DV_GLOBAL double* Gn = gl(a.ring[0] + S.nxt * a.ringStride);
+    const bool mom = !(S.flags & kViewRestart);  // momentum term present
+    // loads of all rows of the thread, issued before any of them is used:
+    // rows the thread does not own read row 0 instead (always present), so
+    // that no load is conditional and no select waits on one; their values
+    // are never used
+    double gk[R][3], gm[R][3];
+    TM rv[R][3], nk[R][3], nm[R][3], fc[R][6];
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       const long q = (long)tile.start + threadIdx.x + u * NT;
@@ -660,15 +722,16 @@ __global__ __launch_bounds__(kTC / R) void kP1(const Args<TM>* __restrict__ A, T
 #pragma unroll
       for (int rr = 0; rr < 3; ++rr) {
         const bool on = valid && (rr == 0 || !bil);
-        const long f = bil ? BROW(a, q - D.nc) : ROW(a, rr, q);
-        gk[u][rr] = on ? Gk[f] : 0.0;
-        rv[u][rr] = on ? (double)a.r[f] : 0.0;
-        nk[u][rr] = on ? Nk[f] : (TM)0;
-        gm[u][rr] = (on && mom) ? Gkm1[f] : 0.0;
-        nm[u][rr] = (on && mom) ? Nkm1[f] : (TM)0;
+        const long f = on ? (bil ? BROW(a, q - D.nc) : ROW(a, rr, q)) : 0;
+        gk[u][rr] = Gk[f];
+        rv[u][rr] = gl(a.r)[f];
+        nk[u][rr] = Nk[f];
+        gm[u][rr] = Gkm1[f];  // read even after a restart (unused then)
+        nm[u][rr] = Nkm1[f];
       }
+      const long qf = (valid && !bil) ? q : 0;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) fc[u][k] = (valid && !bil && k < NF) ? a.frame[k * a.cap + q] : (TM)0;
+      for (int k = 0; k < 6; ++k) fc[u][k] = k < NF ? gl(a.frame)[k * a.cap + qf] : (TM)0;
     }
     const double bp = 1.0 + S.beta, bm = -S.beta;
     const double t = S.t;
@@ -691,7 +754,7 @@ __global__ __launch_bounds__(kTC / R) void kP1(const Args<TM>* __restrict__ A, T
             y[rr] = gk[u][rr];
             ny[rr] = (double)nk[u][rr];
           }
-          gg[rr] = 1.0 * ny[rr] + 1.0 * rv[u][rr];
+          gg[rr] = 1.0 * ny[rr] + 1.0 * (double)rv[u][rr];
           gn[rr] = 1.0 * y[rr] + (-t) * gg[rr];
         }
       }
@@ -703,12 +766,12 @@ __global__ __launch_bounds__(kTC / R) void kP1(const Args<TM>* __restrict__ A, T
           Gn[f] = gn[rr];
           const double dy = gn[rr] - y[rr];
           if (kMixed) {
-            a.delta[f] = (TM)dy;
+            gl(a.delta)[f] = (TM)dy;
             acc[1] += dy * ny[rr];
           } else {
-            acc[0] += gn[rr] * rv[u][rr];
+            acc[0] += gn[rr] * (double)rv[u][rr];
             acc[1] += y[rr] * ny[rr];
-            acc[2] += y[rr] * rv[u][rr];
+            acc[2] += y[rr] * (double)rv[u][rr];
             acc[3] += gg[rr] * dy;
           }
           acc[4] += dy * dy;
@@ -732,13 +795,13 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
   constexpr int NF = kMixed ? 6 : 3;
   TILE_LOOP(T) {
     const Tile tile = rowTile(T, ti_);
-    const ApgdState& S = STATE(T, tile.scene, par ^ 1);
-    if (S.done) continue;
+    const PassView S = loadView(T, tile.scene, par ^ 1);
+    if (viewDone(S)) continue;
     const Args<TM>& a = A[tile.scene];
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
-    const double* Gn = a.ring[S.nxt];
-    TM* Nn = a.rn[S.nxt];
+    const DV_GLOBAL double* Gn = gl(a.ring[0] + S.nxt * a.ringStride);
+    DV_GLOBAL TM* Nn = gl(a.rn[0] + S.nxt * a.ringStride);
     // loads of all contact rows of the thread, then the tmp gathers
     double g[R][3];
     TM dl[R][3], rv[R][3], fc[R][6];
@@ -746,24 +809,25 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       const long q = (long)tile.start + threadIdx.x + u * NT;
-      const bool on = q < D.nc;
+      const long qc = q < D.nc ? q : 0;  // contact 0 stands in for rows not owned
 #pragma unroll
       for (int rr = 0; rr < 3; ++rr) {
-        const long f = ROW(a, rr, q);
-        g[u][rr] = on ? Gn[f] : 0.0;
-        rv[u][rr] = on ? a.r[f] : (TM)0;
-        dl[u][rr] = (on && kMixed) ? a.delta[f] : (TM)0;
+        const long f = ROW(a, rr, qc);
+        g[u][rr] = Gn[f];
+        rv[u][rr] = gl(a.r)[f];
+        dl[u][rr] = kMixed ? gl(a.delta)[f] : (TM)0;
       }
 #pragma unroll
-      for (int k = 0; k < 6; ++k) fc[u][k] = (on && k < NF) ? a.frame[k * a.cap + q] : (TM)0;
-      ca[u] = on ? a.cA[q] : 0;
-      cb[u] = on ? a.cB[q] : 0;
+      for (int k = 0; k < 6; ++k) fc[u][k] = k < NF ? gl(a.frame)[k * a.cap + qc] : (TM)0;
+      // body 0 for rows not owned: the contact arrays may hold no contact
+      ca[u] = q < D.nc ? gl(a.cA)[qc] : 0;
+      cb[u] = q < D.nc ? gl(a.cB)[qc] : 0;
     }
     V4<TM> tA[R], tB[R];
 #pragma unroll
     for (int u = 0; u < R; ++u) {
-      tA[u] = a.tmp[ca[u]];
-      tB[u] = a.tmp[cb[u]];
+      tA[u] = ld4(a.tmp, ca[u]);
+      tB[u] = ld4(a.tmp, cb[u]);
     }
     double acc[kSlots];
     zero(acc);
@@ -776,8 +840,8 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
         const TM nv = bilRow(a, (int)(q - D.nc));
         Nn[f] = nv;
         const double gv = Gn[f];
-        acc[6] += (kMixed ? (double)a.delta[f] : gv) * (double)nv;
-        const double w = (double)nv + (double)a.r[f];
+        acc[6] += (kMixed ? (double)gl(a.delta)[f] : gv) * (double)nv;
+        const double w = (double)nv + (double)gl(a.r)[f];
         double e;
         if (D.nc) {
           const double z = 1.0 * gv + (-D.gdiff) * w;
@@ -826,6 +890,7 @@ __global__ __launch_bounds__(kTD) void kDecide(const Args<TM>* __restrict__ A, T
   }
   if (threadIdx.x == 0) {
     STATE(T, blockIdx.x, par) = S;
+    VIEW(T, blockIdx.x, par) = viewOf(S);
     T.out[blockIdx.x] = S;
   }
 }
@@ -843,7 +908,7 @@ __global__ __launch_bounds__(kTC) void kFin1(const Args<TM>* __restrict__ A, Tab
     const long items = (long)D.nc + D.nbil;
     const long q = (long)tile.start + threadIdx.x;
     const int hatSrc = STATE(T, tile.scene, fin).hatSrc;
-    const double* hatV = hatSrc >= 0 ? a.ring[hatSrc] : a.gHat;
+    const DV_GLOBAL double* hatV = gl(hatSrc >= 0 ? a.ring[hatSrc] : a.gHat);
     double acc[kSlots];
     zero(acc);
     if (q < items) {
@@ -854,9 +919,9 @@ __global__ __launch_bounds__(kTC) void kFin1(const Args<TM>* __restrict__ A, Tab
         const long o = bil ? BROW(a, q - D.nc) : ROW(a, rr, q);
         const long f = bil ? (q - D.nc) : (D.nbil + 3L * q + rr);
         x[rr] = hatV[o];
-        a.gHat[o] = x[rr];
-        a.gammaIO[f] = x[rr];
-        acc[0] += x[rr] * (double)a.r[o];
+        gl(a.gHat)[o] = x[rr];
+        gl(a.gammaIO)[f] = x[rr];
+        acc[0] += x[rr] * (double)gl(a.r)[o];
       }
       if (!bil) {
         TM F[9];
@@ -880,15 +945,15 @@ __global__ __launch_bounds__(kTC) void kFin3(const Args<TM>* __restrict__ A, Tab
     zero(acc);
     if (q < items) {
       if (q >= D.nc) {
-        acc[1] += a.gHat[BROW(a, q - D.nc)] * (double)bilRow(a, (int)(q - D.nc));
+        acc[1] += gl(a.gHat)[BROW(a, q - D.nc)] * (double)bilRow(a, (int)(q - D.nc));
       } else {
         const int c = (int)q;
         TM F[9];
         ldFrame(a, c, D.geomMode, F);
         TM n0, n1, n2;
         contactRows(a, c, F, n0, n1, n2);
-        acc[1] += a.gHat[ROW(a, 0, c)] * (double)n0 + a.gHat[ROW(a, 1, c)] * (double)n1 +
-                  a.gHat[ROW(a, 2, c)] * (double)n2;
+        acc[1] += gl(a.gHat)[ROW(a, 0, c)] * (double)n0 + gl(a.gHat)[ROW(a, 1, c)] * (double)n1 +
+                  gl(a.gHat)[ROW(a, 2, c)] * (double)n2;
       }
     }
     writePartials<kTC>(acc, TILE_PART(a.part3[0], tile));
@@ -1016,6 +1081,7 @@ void Engine::release() {
   retire.release();
   out.release();
   states.release();
+  views.release();
   if (outHost) (void)hipHostFree(outHost);
   outHost = nullptr;
   outHostCap = 0;
@@ -1068,6 +1134,7 @@ void prepareScene(Scene& s, Args<TM>& a) {
   a.rFlat = s.r.p;
   double* W = s.work.p;
   for (int k = 0; k < 4; ++k) a.ring[k] = W + k * RsCap;
+  a.ringStride = RsCap;
   a.gHat = W + 4 * RsCap;
   for (int k = 0; k < 4; ++k) a.rn[k] = WM + k * RsCap;
   a.r = WM + 4 * RsCap;
@@ -1141,6 +1208,7 @@ Tables tablesOf(Engine& E, TileSet& set) {
   T.slots = set.slots.p;
   T.out = reinterpret_cast<ApgdState*>(E.out.p);
   T.st = reinterpret_cast<ApgdState*>(E.states.p);
+  T.pv = reinterpret_cast<PassView*>(E.views.p);
   T.single = E.nSlots == 1 ? 1 : 0;
   return T;
 }
@@ -1288,6 +1356,7 @@ void engineOpen(Engine& E, int nSlots, bool mixed, int numCU, bool noGraph, int 
   else openT<double>(E, nSlots);
   E.out.reserve(sizeof(ApgdState) * nSlots);
   E.states.reserve(2 * sizeof(ApgdState) * nSlots);
+  E.views.reserve(2 * sizeof(PassView) * nSlots);
   if (E.outHostCap < nSlots) {
     if (E.outHost) (void)hipHostFree(E.outHost);
     E.outHost = nullptr;

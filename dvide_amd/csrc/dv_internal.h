@@ -117,6 +117,7 @@ struct Engine {
   TileSet active, admit, retire;
   DevBuf<char> out;     // ApgdState[nSlots]
   DevBuf<char> states;  // ApgdState[nSlots][2], by pass parity
+  DevBuf<char> views;   // PassView[nSlots][2]: what the pass kernels read of the states
   void* outHost = nullptr;  // pinned copy of out
   int outHostCap = 0;
   GraphCache graph;
