@@ -430,17 +430,13 @@ __device__ __forceinline__ TM bodyMinv(const Args<TM>& a, int g, int which) {
 
 template <typename TM>
 __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tables T, int which, int sIdx) {
-  for (int ti = blockIdx.x, nt = T.counts[1]; ti < nt; ti += gridDim.x) {
-    const Tile tile = bodyTile(T, ti);
-    const PassView S = loadView(T, tile.scene, sIdx);
-    if (which == 2 && viewDone(S)) continue;
-    const Args<TM>& a = A[tile.scene];
+  auto tileBody = [&](const Tile tile, const PassView& S, const Args<TM>& a) {
     const int g = tile.start + threadIdx.x;
-    if (g >= a.dims.nb) continue;
+    if (g >= a.dims.nb) return;
     const DV_GLOBAL double* X = bodySource(a, which, S);
     const long bil0 = 3L * a.cap;
     const int k0 = gl(a.iStart)[g], k1 = gl(a.iStart)[g + 1];
-    if (k1 - k0 > kHeavyDegree) continue;  // reduced below by a workgroup
+    if (k1 - k0 > kHeavyDegree) return;  // reduced below by a workgroup
     const TM mi = bodyMinv(a, g, which);
     V4<TM> out;
     out.x = 0; out.y = 0; out.z = 0; out.w = 0;
@@ -473,6 +469,20 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
       out.x = mi * s0; out.y = mi * s1; out.z = mi * s2;
     }
     st4(a.tmp, g, out);
+  };
+  if (T.single) {  // as in kP1
+    const PassView S = loadView(T, 0, sIdx);
+    const Args<TM>& a = A[0];
+    const int nt = T.counts[1];
+    if (!(which == 2 && viewDone(S)))
+      for (int ti = blockIdx.x; ti < nt; ti += gridDim.x) tileBody(Tile{0, ti * kTB}, S, a);
+  } else {
+    for (int ti = blockIdx.x, nt = T.counts[1]; ti < nt; ti += gridDim.x) {
+      const Tile tile = bodyTile(T, ti);
+      const PassView S = loadView(T, tile.scene, sIdx);
+      if (which == 2 && viewDone(S)) continue;
+      tileBody(tile, S, A[tile.scene]);
+    }
   }
   for (int hq = blockIdx.x, nh = T.counts[2]; hq < nh; hq += gridDim.x) {
     const int2 ref = T.heavy[hq];
@@ -696,11 +706,7 @@ __global__ __launch_bounds__(kTC / R) void kP1(const Args<TM>* __restrict__ A, T
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   constexpr int NT = kTC / R;
   constexpr int NF = kMixed ? 6 : 3;  // stored frame components
-  TILE_LOOP(T) {
-    const Tile tile = rowTile(T, ti_);
-    const PassView S = loadView(T, tile.scene, par ^ 1);
-    if (viewDone(S)) continue;
-    const Args<TM>& a = A[tile.scene];
+  auto tileBody = [&](const Tile tile, const PassView& S, const Args<TM>& a) {
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
     const DV_GLOBAL double* Gk = gl(a.ring[0] + S.cur * a.ringStride);
@@ -785,6 +791,22 @@ __global__ __launch_bounds__(kTC / R) void kP1(const Args<TM>* __restrict__ A, T
       }
     }
     writePartials<NT>(acc, TILE_PART(a.part1[par], tile));
+  };
+  if (T.single) {
+    // one slot: its view, table entry and tile count do not depend on the
+    // tile, so they are requested together before the tile loop
+    const PassView S = loadView(T, 0, par ^ 1);
+    const Args<TM>& a = A[0];
+    const int nt = T.counts[0];
+    if (viewDone(S)) return;
+    for (int ti = blockIdx.x; ti < nt; ti += gridDim.x) tileBody(Tile{0, ti * kTC}, S, a);
+    return;
+  }
+  TILE_LOOP(T) {
+    const Tile tile = rowTile(T, ti_);
+    const PassView S = loadView(T, tile.scene, par ^ 1);
+    if (viewDone(S)) continue;
+    tileBody(tile, S, A[tile.scene]);
   }
 }
 
@@ -793,11 +815,7 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   constexpr int NT = kTC / R;
   constexpr int NF = kMixed ? 6 : 3;
-  TILE_LOOP(T) {
-    const Tile tile = rowTile(T, ti_);
-    const PassView S = loadView(T, tile.scene, par ^ 1);
-    if (viewDone(S)) continue;
-    const Args<TM>& a = A[tile.scene];
+  auto tileBody = [&](const Tile tile, const PassView& S, const Args<TM>& a) {
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
     const DV_GLOBAL double* Gn = gl(a.ring[0] + S.nxt * a.ringStride);
@@ -871,6 +889,20 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
       }
     }
     writePartials<NT>(acc, TILE_PART(a.part3[par], tile));
+  };
+  if (T.single) {  // as in kP1
+    const PassView S = loadView(T, 0, par ^ 1);
+    const Args<TM>& a = A[0];
+    const int nt = T.counts[0];
+    if (viewDone(S)) return;
+    for (int ti = blockIdx.x; ti < nt; ti += gridDim.x) tileBody(Tile{0, ti * kTC}, S, a);
+    return;
+  }
+  TILE_LOOP(T) {
+    const Tile tile = rowTile(T, ti_);
+    const PassView S = loadView(T, tile.scene, par ^ 1);
+    if (viewDone(S)) continue;
+    tileBody(tile, S, A[tile.scene]);
   }
 }
 
