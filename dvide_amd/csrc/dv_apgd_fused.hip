@@ -418,31 +418,37 @@ __device__ __forceinline__ const DV_GLOBAL double* bodySource(const Args<TM>& a,
 }
 
 template <typename TM>
-__device__ __forceinline__ TM bodyMinv(const Args<TM>& a, int g, int which) {
+__device__ __forceinline__ TM bodyMinv(const Args<TM>& a, int g, int which, bool own = true) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   if (kMixed && which == 0) {
     const TM mi = (TM)gl(a.minv64)[g];
-    gl(a.minv)[g] = mi;
+    if (own) gl(a.minv)[g] = mi;
     return mi;
   }
   return gl(a.minv)[g];
 }
 
-template <typename TM>
-__global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tables T, int which, int sIdx) {
+template <typename TM, int which>
+__global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tables T, int sIdx) {
   auto tileBody = [&](const Tile tile, const PassView& S, const Args<TM>& a) {
+    // no lane leaves early: lanes past the last body read body 0's entries
+    // (unused), so that the index loads and M^-1 are all requested at once
     const int g = tile.start + threadIdx.x;
-    if (g >= a.dims.nb) return;
+    const bool own = g < a.dims.nb;
+    const int gs = own ? g : 0;
+    const int k0 = gl(a.iStart)[gs], k1r = gl(a.iStart)[gs + 1];
+    const int b0 = gl(a.bilIncStart)[gs], b1r = gl(a.bilIncStart)[gs + 1];
+    const TM mi = bodyMinv(a, gs, which, own);
+    const bool heavy = k1r - k0 > kHeavyDegree;  // reduced below by a workgroup
+    const bool sums = own && !heavy && mi != (TM)0;
+    const int k1 = sums ? k1r : k0, b1 = sums ? b1r : b0;
     const DV_GLOBAL double* X = bodySource(a, which, S);
     const long bil0 = 3L * a.cap;
-    const int k0 = gl(a.iStart)[g], k1 = gl(a.iStart)[g + 1];
-    if (k1 - k0 > kHeavyDegree) return;  // reduced below by a workgroup
-    const TM mi = bodyMinv(a, g, which);
     V4<TM> out;
     out.x = 0; out.y = 0; out.z = 0; out.w = 0;
-    if (mi != (TM)0) {
+    {
       TM s0 = 0, s1 = 0, s2 = 0;
-      for (int k = gl(a.bilIncStart)[g]; k < gl(a.bilIncStart)[g + 1]; ++k) {
+      for (int k = b0; k < b1; ++k) {
         const int e = gl(a.bilInc)[k];
         const TM t = (TM)((e & 1) ? -X[bil0 + (e >> 3)] : X[bil0 + (e >> 3)]);
         const int comp = (e >> 1) & 3;
@@ -468,7 +474,7 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
       }
       out.x = mi * s0; out.y = mi * s1; out.z = mi * s2;
     }
-    st4(a.tmp, g, out);
+    if (own && !heavy) st4(a.tmp, g, out);
   };
   if (T.single) {  // as in kP1
     const PassView S = loadView(T, 0, sIdx);
@@ -912,6 +918,22 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
 template <typename TM>
 __global__ __launch_bounds__(kTD) void kDecide(const Args<TM>* __restrict__ A, Tables T, int par) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
+  if (T.single) {
+    // one slot, always active: the table entry and the state are requested
+    // together, and the partials are reduced without waiting for the done
+    // flag (after the solve has ended the totals are simply not used)
+    const Args<TM>& a = A[0];
+    ApgdState S = STATE(T, 0, par ^ 1);
+    double tot[2 * kSlots];
+    reduceAll(a.part1[par], a.part3[par], a.nTiles, tot);
+    if (!S.done) decideStep<kMixed>(S, tot, a.dims);
+    if (threadIdx.x == 0) {
+      STATE(T, 0, par) = S;
+      VIEW(T, 0, par) = viewOf(S);
+      T.out[0] = S;
+    }
+    return;
+  }
   const Args<TM>& a = A[blockIdx.x];  // one workgroup per slot
   if (!a.active) return;
   ApgdState S = STATE(T, blockIdx.x, par ^ 1);
@@ -1074,7 +1096,7 @@ void enqueuePasses(const Plan<TM>& P, hipStream_t st, int n, hipEvent_t* ev, int
       if (timed) DV_HIP(hipEventRecord(ev[0], st));
       for (int r = 0; r < reps; ++r) {
         if (kern == 1) launchP1<TM>(P, st, par);
-        if (kern == 2) hipLaunchKernelGGL(kBody<TM>, dim3(P.gb), dim3(kTB), 0, st, P.A, P.T, 2, par ^ 1);
+        if (kern == 2) hipLaunchKernelGGL((kBody<TM, 2>), dim3(P.gb), dim3(kTB), 0, st, P.A, P.T, par ^ 1);
         if (kern == 3) launchP3<TM>(P, st, par);
         if (kern == 4) hipLaunchKernelGGL(kDecide<TM>, dim3(P.nSlots), dim3(kTD), 0, st, P.A, P.T, par);
       }
@@ -1300,9 +1322,9 @@ void startAdmitted(Engine& E, hipStream_t st) {
   const int n = (int)E.admit.hSlots.size();
   // setup (APGD.cu:195-223)
   hipLaunchKernelGGL(kSetup1<TM>, dim3(gc), dim3(kTC), 0, st, A, T);
-  hipLaunchKernelGGL(kBody<TM>, dim3(gb), dim3(kTB), 0, st, A, T, 0, 0);
+  hipLaunchKernelGGL((kBody<TM, 0>), dim3(gb), dim3(kTB), 0, st, A, T, 0);
   hipLaunchKernelGGL(kSetup3<TM>, dim3(gc), dim3(kTC), 0, st, A, T);
-  hipLaunchKernelGGL(kBody<TM>, dim3(gb), dim3(kTB), 0, st, A, T, 1, 0);
+  hipLaunchKernelGGL((kBody<TM, 1>), dim3(gb), dim3(kTB), 0, st, A, T, 0);
   hipLaunchKernelGGL(kSetup5<TM>, dim3(gc), dim3(kTC), 0, st, A, T);
   hipLaunchKernelGGL(kSetupDecide<TM>, dim3(n), dim3(kTD), 0, st, A, T);
   E.admit.hSlots.clear();
@@ -1343,7 +1365,7 @@ void retireSlots(Engine& E, const std::vector<int>& slots, hipStream_t st) {
   grids<TM>(E, E.retire, gc, gb);
   const int fin = 1;  // a chunk's last pass (parity 1) decides into st[1]
   hipLaunchKernelGGL(kFin1<TM>, dim3(gc), dim3(kTC), 0, st, A, T, fin);
-  hipLaunchKernelGGL(kBody<TM>, dim3(gb), dim3(kTB), 0, st, A, T, 3, 0);
+  hipLaunchKernelGGL((kBody<TM, 3>), dim3(gb), dim3(kTB), 0, st, A, T, 0);
   hipLaunchKernelGGL(kFin3<TM>, dim3(gc), dim3(kTC), 0, st, A, T);
   hipLaunchKernelGGL(kFinDecide<TM>, dim3((int)slots.size()), dim3(kTD), 0, st, A, T, fin);
   ApgdState* outH = reinterpret_cast<ApgdState*>(E.outHost);
