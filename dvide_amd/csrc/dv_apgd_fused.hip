@@ -279,6 +279,45 @@ __device__ void reduceAll(const double* part1, const double* part3, int ng, doub
   for (int s = 0; s < 2 * kSlots; ++s) out[s] = (s % kSlots == kSlots - 1) ? waveMax(v[s]) : waveSum(v[s]);
 }
 
+// reduceAll for the pass decision, which needs only P1's slots 0..5 and P3's
+// slots 6 (sum) and 7 (max): eight values per tile instead of sixteen, so a
+// wavefront requests 512 tiles' worth at once.  Per slot the order of the
+// additions is reduceAll's, so the totals are the same; the other slots of
+// out are zero.
+__device__ void reducePass(const double* part1, const double* part3, int ng, double* out) {
+  constexpr int K = 8;  // tiles per lane per round
+  const int lane = threadIdx.x & 63;
+  double v[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) v[s] = 0.0;
+  for (int base = 0; base < ng; base += 64 * K) {
+    double x[K][8];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int w = base + lane + 64 * k;
+      const int ws = w < ng ? w : 0;
+#pragma unroll
+      for (int s = 0; s < 6; ++s) x[k][s] = gl(part1)[(long)ws * kSlots + s];
+      x[k][6] = gl(part3)[(long)ws * kSlots + 6];
+      x[k][7] = gl(part3)[(long)ws * kSlots + 7];
+    }
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const double xv = (base + lane + 64 * k) < ng ? x[k][s] : 0.0;
+        v[s] = s == 7 ? fmax(v[s], xv) : v[s] + xv;
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 2 * kSlots; ++s) out[s] = 0.0;
+#pragma unroll
+  for (int s = 0; s < 6; ++s) out[s] = waveSum(v[s]);
+  out[kSlots + 6] = waveSum(v[6]);
+  out[kSlots + 7] = waveMax(v[7]);
+}
+
 // ---------------------------------------------------------------------------
 // contact frames and row helpers
 
@@ -925,7 +964,7 @@ __global__ __launch_bounds__(kTD) void kDecide(const Args<TM>* __restrict__ A, T
     const Args<TM>& a = A[0];
     ApgdState S = STATE(T, 0, par ^ 1);
     double tot[2 * kSlots];
-    reduceAll(a.part1[par], a.part3[par], a.nTiles, tot);
+    reducePass(a.part1[par], a.part3[par], a.nTiles, tot);
     if (!S.done) decideStep<kMixed>(S, tot, a.dims);
     if (threadIdx.x == 0) {
       STATE(T, 0, par) = S;
@@ -939,7 +978,7 @@ __global__ __launch_bounds__(kTD) void kDecide(const Args<TM>* __restrict__ A, T
   ApgdState S = STATE(T, blockIdx.x, par ^ 1);
   if (!S.done) {
     double tot[2 * kSlots];
-    reduceAll(a.part1[par], a.part3[par], a.nTiles, tot);
+    reducePass(a.part1[par], a.part3[par], a.nTiles, tot);
     decideStep<kMixed>(S, tot, a.dims);
   }
   if (threadIdx.x == 0) {
