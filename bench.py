@@ -126,6 +126,36 @@ def copy_calibration(nbytes=1 << 30):
     print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
 
 
+def _short_kernel_name(name):
+    name = name.replace("dv::(anonymous namespace)::", "").replace("void ", "")
+    return name.split("(")[0]
+
+
+def summarize_rocprof(path):
+    """Markdown table of a rocprofv3 --output-format csv file: per kernel, the
+    launch count and durations (kernel_stats.csv), or the median value of
+    each counter over the launches (counter_collection.csv)."""
+    import csv
+
+    rows = list(csv.DictReader(open(path)))
+    if rows and "AverageNs" in rows[0]:
+        print("| kernel | launches | avg us | min us | max us | total ms |")
+        print("|---|---|---|---|---|---|")
+        for r in rows[:16]:
+            print(f"| `{_short_kernel_name(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                  f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | "
+                  f"{float(r['TotalDurationNs']) / 1e6:.2f} |")
+        return
+    vals = {}
+    for r in rows:
+        key = (_short_kernel_name(r["Kernel_Name"]), r["Counter_Name"])
+        vals.setdefault(key, []).append(float(r["Counter_Value"]))
+    print("| kernel | counter | launches | median | mean |")
+    print("|---|---|---|---|---|")
+    for (k, c), v in sorted(vals.items(), key=lambda kv: -len(kv[1])):
+        print(f"| `{k}` | {c} | {len(v)} | {float(np.median(v)):.0f} | {float(np.mean(v)):.0f} |")
+
+
 def lid_pressed(steps=5):
     """The deposit under a free lid as heavy as the spheres (the reference's
     compaction driver): per-step APGD iterations and times of both GPU
@@ -307,7 +337,12 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served request stream")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB copy kernel (for scaling rocprofv3 byte counters)")
+    ap.add_argument("--summarize", metavar="CSV",
+                    help="print a markdown summary of a rocprofv3 csv (kernel_stats or counter_collection) and exit")
     args = ap.parse_args()
+    if args.summarize:
+        summarize_rocprof(args.summarize)
+        return
     if args.calibrate_copy:
         copy_calibration()
         return
@@ -322,6 +357,13 @@ def main():
     cap = args.max_iterations
     mixed = run_gpu(scene, system.SOLVER_MIXED, args.steps, args.warmup, profile_kernel=system.PROFILE_P1,
                     max_iterations=cap)
+    # burst-timed averages of the body pass P2 and the contact pass P3 (two
+    # untimed steps each from the same state): the Schur-product phase
+    kernel_us = {"P1": None, "P2": None, "P3": None}
+    for pk, name in ((system.PROFILE_P2, "P2"), (system.PROFILE_P3, "P3")):
+        r = run_gpu(scene, system.SOLVER_MIXED, 2, 0, profile_kernel=pk, max_iterations=cap)
+        if r["kernel_count"]:
+            kernel_us[name] = round(1e3 * r["kernel_ms"] / r["kernel_count"], 3)
     others = {}
     if not args.only_mixed:
         others["fp64"] = run_gpu(scene, system.SOLVER_FUSED, args.steps, args.warmup, max_iterations=cap)
@@ -385,6 +427,20 @@ def main():
             "bytes_per_contact": p1_bytes_per_contact(True),
         },
     }
+    kernel_us["P1"] = round(k_us, 3)
+    line["kernel_us"] = kernel_us
+    if kernel_us["P2"] and kernel_us["P3"]:
+        # the algorithmic bytes of one Schur product N x with a 3x3 frame per
+        # contact, mixed precision: 112 B per contact + 28 B per body
+        b_n = 112 * nc + 28 * mixed["nb"]
+        t_n = kernel_us["P2"] + kernel_us["P3"]
+        line["spmv_phase"] = {
+            "kernels": "kBody (D^T gather, M^-1) + kP3 (D gather, residual)",
+            "bytes": b_n,
+            "us": round(t_n, 3),
+            "achieved_gbs": round(b_n / (t_n * 1e-6) / 1e9, 1),
+            "frac": round(b_n / (t_n * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+        }
     def us_per_iteration(r):
         return round(1e3 * float(np.sum(r["solve_ms"])) / max(int(np.sum(r["iterations"])), 1), 2)
 

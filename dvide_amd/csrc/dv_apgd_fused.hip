@@ -1118,27 +1118,61 @@ void launchP3(const Plan<TM>& P, hipStream_t st, int par) {
 
 constexpr int kProfBurst = 16;  // timed back-to-back launches of the profiled kernel
 
+template <typename TM>
+void launchPass(const Plan<TM>& P, hipStream_t st, int kern, int par) {
+  if (kern == 1) launchP1<TM>(P, st, par);
+  if (kern == 2) hipLaunchKernelGGL((kBody<TM, 2>), dim3(P.gb), dim3(kTB), 0, st, P.A, P.T, par ^ 1);
+  if (kern == 3) hipLaunchKernelGGL((kP3<TM, 1>), dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
+  if (kern == 4) hipLaunchKernelGGL(kDecide<TM>, dim3(P.nSlots), dim3(kTD), 0, st, P.A, P.T, par);
+}
+
+// kProfBurst launches of pass kernel `kern` (P1, P2 or P3) between ev[0] and
+// ev[1], as a captured graph like the passes themselves, so that the average
+// is the kernel's duration in a graph rather than that plus a stream
+// launch's latency.  The graph is kept for the next solve of the same plan.
+template <typename TM>
+void timedBurst(const Plan<TM>& P, hipStream_t st, int kern, int par, hipEvent_t* ev, GraphCache& cache) {
+  struct Key {
+    Plan<TM> P;
+    int kern, par;
+  } key;
+  std::memset(&key, 0, sizeof(key));
+  key.P = P;
+  key.kern = kern;
+  key.par = par;
+  if (!cache.matches(&key, sizeof(key))) {
+    cache.reset();
+    hipGraph_t graph;
+    DV_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int r = 0; r < kProfBurst; ++r) launchPass<TM>(P, st, kern, par);
+    DV_HIP(hipStreamEndCapture(st, &graph));
+    hipError_t e = hipGraphInstantiate(&cache.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) throw HipError{e, "hipGraphInstantiate(timed burst)"};
+    cache.setKey(&key, sizeof(key));
+  }
+  DV_HIP(hipEventRecord(ev[0], st));
+  DV_HIP(hipGraphLaunch(cache.exec, st));
+  DV_HIP(hipEventRecord(ev[1], st));
+}
+
 // One chunk of n (even) passes, pass k with parity k & 1.  With `ev` (stream
 // launches only: events recorded inside a captured graph cannot be timed),
-// kernel `prof` of the chunk's first pass runs kProfBurst times back to back
-// between ev[0] and ev[1], so that the launch latency of a lone kernel does
-// not enter its average.  P1, P2 and P3 read nothing they write, so the
-// repeats reproduce the same outputs; the decision kernel is not repeatable
-// and is timed alone.
+// kernel `prof` of the chunk's first pass runs as a timed burst (P1, P2 and
+// P3 read nothing they write, so the repeats reproduce the same outputs);
+// the decide kernel, which is not repeatable, is timed alone.
 template <typename TM>
-void enqueuePasses(const Plan<TM>& P, hipStream_t st, int n, hipEvent_t* ev, int prof) {
+void enqueuePasses(const Plan<TM>& P, hipStream_t st, int n, hipEvent_t* ev, int prof, GraphCache* burst) {
   for (int k = 0; k < n; ++k) {
     const int par = k & 1;
     for (int kern = 1; kern <= 4; ++kern) {
       const bool timed = ev && prof == kern && k == 0;
-      const int reps = timed && kern != 4 ? kProfBurst : 1;
-      if (timed) DV_HIP(hipEventRecord(ev[0], st));
-      for (int r = 0; r < reps; ++r) {
-        if (kern == 1) launchP1<TM>(P, st, par);
-        if (kern == 2) hipLaunchKernelGGL((kBody<TM, 2>), dim3(P.gb), dim3(kTB), 0, st, P.A, P.T, par ^ 1);
-        if (kern == 3) launchP3<TM>(P, st, par);
-        if (kern == 4) hipLaunchKernelGGL(kDecide<TM>, dim3(P.nSlots), dim3(kTD), 0, st, P.A, P.T, par);
+      if (timed && kern != 4) {
+        timedBurst<TM>(P, st, kern, par, ev, *burst);
+        continue;
       }
+      if (timed) DV_HIP(hipEventRecord(ev[0], st));
+      launchPass<TM>(P, st, kern, par);
       if (timed) DV_HIP(hipEventRecord(ev[1], st));
     }
   }
@@ -1179,6 +1213,7 @@ void Engine::release() {
   outHost = nullptr;
   outHostCap = 0;
   graph.reset();
+  burst.reset();
   hostTable.clear();
   slotScene.clear();
   nSlots = 0;
@@ -1374,13 +1409,13 @@ template <typename TM>
 void runChunk(Engine& E, hipStream_t st, hipEvent_t* ev, int prof) {
   const Plan<TM> P = activePlan<TM>(E);
   if (E.noGraph || ev) {
-    enqueuePasses<TM>(P, st, kPassesPerChunk, ev, prof);
+    enqueuePasses<TM>(P, st, kPassesPerChunk, ev, prof, &E.burst);
   } else {
     if (!E.graph.matches(&P, sizeof(P))) {
       E.graph.reset();
       hipGraph_t graph;
       DV_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      enqueuePasses<TM>(P, st, kPassesPerChunk, nullptr, 0);
+      enqueuePasses<TM>(P, st, kPassesPerChunk, nullptr, 0, nullptr);
       DV_HIP(hipStreamEndCapture(st, &graph));
       hipError_t e = hipGraphInstantiate(&E.graph.exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);

@@ -121,6 +121,7 @@ struct Engine {
   void* outHost = nullptr;  // pinned copy of out
   int outHostCap = 0;
   GraphCache graph;
+  GraphCache burst;     // the timed burst of one pass kernel (DV_PARAM_PROFILE)
   int numCU = 0;
   bool noGraph = false;
   long launched = 0;    // passes launched since the last reset

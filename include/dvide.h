@@ -66,11 +66,13 @@ typedef struct dv_scene dv_scene;
 #define DV_PARAM_GRID 13      /* unused (accepted for compatibility) */
 #define DV_PARAM_PROFILE 14   /* time one kernel of the graph-driven solvers with HIP
                                  events: 0 off, 1 P1 (contacts), 2 P2 (bodies),
-                                 3 P3 (contacts), 4 the decide kernel.  The first chunk
-                                 of 16 passes of each solve is then launched on the
-                                 scene's stream, and in its first pass that kernel runs
-                                 16 times back to back between two events (the decide
-                                 kernel once); the rest run in the captured graph */
+                                 3 P3 (contacts), 4 the decide kernel.  One chunk of 16
+                                 passes per solve (the first after the first chunk whose
+                                 entry state has the momentum term on) is then launched
+                                 on the scene's stream, and in its first pass that
+                                 kernel runs as a captured graph of 16 back-to-back
+                                 launches between two events (the decide kernel once,
+                                 alone); the rest run in the captured chunk graph */
 #define DV_PARAM_ROWS_PER_THREAD 15 /* contact kernels of the graph-driven solvers: 1 or 2
                                        contacts per thread (a tuning choice; results agree
                                        to rounding, each choice is deterministic) */
