@@ -375,14 +375,15 @@ def main():
     k_us = 1e3 * mixed["kernel_ms"] / max(mixed["kernel_count"], 1)
     alg = p1_bytes_per_contact(True) * nc
     achieved = alg / (k_us * 1e-6) / 1e9 if k_us > 0 else 0.0
-    # HBM bytes per P1 launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    # summary (scaled by a copy kernel); measured on the same workload
+    # HBM bytes per P1 launch: the per-contact figure of the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary (scaled by a copy kernel) times this
+    # workload's contacts
     traffic = None
     tpath = os.path.join(_HERE, "profiles", "apgd_traffic_compaction_112k_mixed.md")
     if os.path.exists(tpath):
         for ln in open(tpath):
-            if ln.startswith("p1_traffic_bytes_per_launch:"):
-                traffic = int(float(ln.split(":")[1]))
+            if ln.startswith("p1_traffic_bytes_per_contact:"):
+                traffic = int(round(float(ln.split(":")[1]) * nc))
 
     def lat_ms(r):
         return dict(p50=round(1e3 * float(np.percentile(r["lat"], 50)), 3),
