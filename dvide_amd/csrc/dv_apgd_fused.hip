@@ -865,7 +865,9 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
     const long items = (long)D.nc + D.nbil;
     const DV_GLOBAL double* Gn = gl(a.ring[0] + S.nxt * a.ringStride);
     DV_GLOBAL TM* Nn = gl(a.rn[0] + S.nxt * a.ringStride);
-    // loads of all contact rows of the thread, then the tmp gathers
+    // loads of all contact rows of the thread, the body ids first: loads
+    // return in order, so the tmp gathers, which need the ids, wait only for
+    // them and go out while the rest are still in flight
     double g[R][3];
     TM dl[R][3], rv[R][3], fc[R][6];
     int ca[R], cb[R];
@@ -873,6 +875,14 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
     for (int u = 0; u < R; ++u) {
       const long q = (long)tile.start + threadIdx.x + u * NT;
       const long qc = q < D.nc ? q : 0;  // contact 0 stands in for rows not owned
+      // body 0 for rows not owned: the contact arrays may hold no contact
+      ca[u] = q < D.nc ? gl(a.cA)[qc] : 0;
+      cb[u] = q < D.nc ? gl(a.cB)[qc] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const long q = (long)tile.start + threadIdx.x + u * NT;
+      const long qc = q < D.nc ? q : 0;
 #pragma unroll
       for (int rr = 0; rr < 3; ++rr) {
         const long f = ROW(a, rr, qc);
@@ -882,9 +892,6 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
       }
 #pragma unroll
       for (int k = 0; k < 6; ++k) fc[u][k] = k < NF ? gl(a.frame)[k * a.cap + qc] : (TM)0;
-      // body 0 for rows not owned: the contact arrays may hold no contact
-      ca[u] = q < D.nc ? gl(a.cA)[qc] : 0;
-      cb[u] = q < D.nc ? gl(a.cB)[qc] : 0;
     }
     V4<TM> tA[R], tB[R];
 #pragma unroll
