@@ -83,7 +83,7 @@ def run_gpu(scene, solver, steps, warmup, profile_kernel=0, max_iterations=None)
             g.step()
         g.setProfileKernel(profile_kernel)
         hip.hipDeviceSynchronize()
-        lat, its, res, nc, passes, solve, kms, kcnt = [], [], [], [], [], [], 0.0, 0
+        lat, its, res, nc, passes, solve, kms, kcnt, kstep = [], [], [], [], [], [], 0.0, 0, []
         t0 = time.perf_counter()
         for _ in range(steps):
             ts = time.perf_counter()
@@ -97,13 +97,14 @@ def run_gpu(scene, solver, steps, warmup, profile_kernel=0, max_iterations=None)
             solve.append(st.ms_solve)
             kms += st.prof_kernel_ms
             kcnt += st.prof_kernel_count
+            kstep.append(round(1e3 * st.prof_kernel_ms / st.prof_kernel_count, 3) if st.prof_kernel_count else None)
         hip.hipDeviceSynchronize()
         elapsed = time.perf_counter() - t0
         last = g.stats()
         return dict(
             elapsed=elapsed, lat=np.array(lat), iterations=np.array(its), residual=np.array(res),
             contacts=np.array(nc), passes=np.array(passes), solve_ms=np.array(solve), kernel_ms=kms,
-            kernel_count=kcnt,
+            kernel_count=kcnt, kernel_us_steps=kstep,
             phases=dict(collision=last.ms_collision, assembly=last.ms_assembly, solve=last.ms_solve,
                         update=last.ms_update),
             nb=g.numBodies, v=g.v,
@@ -425,6 +426,7 @@ def main():
             "traffic": traffic,
             "avg_us": round(k_us, 3),
             "launches_timed": mixed["kernel_count"],
+            "avg_us_per_step": mixed["kernel_us_steps"],
             "bytes_per_contact": p1_bytes_per_contact(True),
         },
     }

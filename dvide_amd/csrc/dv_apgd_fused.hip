@@ -1157,6 +1157,7 @@ void timedBurst(const Plan<TM>& P, hipStream_t st, int kern, int par, hipEvent_t
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) throw HipError{e, "hipGraphInstantiate(timed burst)"};
     cache.setKey(&key, sizeof(key));
+    DV_HIP(hipGraphUpload(cache.exec, st));  // outside the timed region
   }
   DV_HIP(hipEventRecord(ev[0], st));
   DV_HIP(hipGraphLaunch(cache.exec, st));
@@ -1218,6 +1219,10 @@ void Engine::release() {
   views.release();
   if (outHost) (void)hipHostFree(outHost);
   outHost = nullptr;
+  for (hipEvent_t& e : polled) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
   outHostCap = 0;
   graph.reset();
   burst.reset();
@@ -1495,7 +1500,7 @@ void engineOpen(Engine& E, int nSlots, bool mixed, int numCU, bool noGraph, int 
   if (E.outHostCap < nSlots) {
     if (E.outHost) (void)hipHostFree(E.outHost);
     E.outHost = nullptr;
-    DV_HIP(hipHostMalloc(&E.outHost, sizeof(ApgdState) * nSlots, hipHostMallocDefault));
+    DV_HIP(hipHostMalloc(&E.outHost, 2 * sizeof(ApgdState) * nSlots, hipHostMallocDefault));
     E.outHostCap = nSlots;
   }
 }
@@ -1567,24 +1572,46 @@ void solveAll(Engine& E, const std::vector<Scene*>& scenes, hipStream_t st, int 
   const long passCap = (long)maxIt * 64 + 4096;
   int passesBefore = 0;
   ApgdState* outH = reinterpret_cast<ApgdState*>(E.outHost);
-  bool sampled = false;
-  for (;;) {
-    const bool timed = ev && !sampled && E.launched > 0 && !outH[0].restart && !outH[0].done;
-    sampled = sampled || timed;
+  ApgdState* buf[2] = {outH, outH + E.nSlots};
+  for (int b = 0; b < 2; ++b)
+    if (!E.polled[b]) DV_HIP(hipEventCreateWithFlags(&E.polled[b], hipEventDisableTiming));
+  // a chunk, then the copy of the states after it into buf[b]
+  auto launch = [&](int b, bool timed) {
     runChunk<TM>(E, st, timed ? ev : nullptr, prof);
-    DV_HIP(hipMemcpyAsync(outH, E.out.p, sizeof(ApgdState) * E.nSlots, hipMemcpyDeviceToHost, st));
-    DV_HIP(hipStreamSynchronize(st));
-    if (timed && profStats && outH[0].passes > passesBefore) {  // the first pass did work
+    DV_HIP(hipMemcpyAsync(buf[b], E.out.p, sizeof(ApgdState) * E.nSlots, hipMemcpyDeviceToHost, st));
+    DV_HIP(hipEventRecord(E.polled[b], st));
+  };
+  bool sampled = false, curTimed = false;
+  int cur = 0;
+  launch(cur, false);
+  for (;;) {
+    // The next chunk is queued before this one's states are read, so the GPU
+    // does not wait for the host between chunks (after the solve ends, the
+    // queued chunk's passes exit at once).  While the profiling sample is
+    // still to be taken, the next chunk's entry state must be known first.
+    const bool ahead = !(ev && !sampled);
+    if (ahead) launch(cur ^ 1, false);
+    DV_HIP(hipEventSynchronize(E.polled[cur]));
+    const ApgdState* last = buf[cur];
+    if (curTimed && profStats && last[0].passes > passesBefore) {  // the first pass did work
       float ms = 0;
       DV_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
       profStats->prof_kernel_ms += ms;
       profStats->prof_kernel_count += prof == 4 ? 1 : kProfBurst;
     }
-    passesBefore = outH[0].passes;
+    passesBefore = last[0].passes;
     bool all = true;
-    for (int i = 0; i < n && all; ++i) all = outH[i].done != 0;
+    for (int i = 0; i < n && all; ++i) all = last[i].done != 0;
     if (all) break;
     if (E.launched > passCap) throw DvError{DV_ERR_DIVERGED, "APGD exceeded its pass budget"};
+    cur ^= 1;
+    curTimed = false;
+    if (!ahead) {
+      const bool timed = ev && !sampled && !last[0].restart && !last[0].done;
+      sampled = sampled || timed;
+      launch(cur, timed);
+      curTimed = timed;
+    }
   }
   std::vector<int> slots(n);
   for (int i = 0; i < n; ++i) slots[i] = i;

@@ -118,7 +118,8 @@ struct Engine {
   DevBuf<char> out;     // ApgdState[nSlots]
   DevBuf<char> states;  // ApgdState[nSlots][2], by pass parity
   DevBuf<char> views;   // PassView[nSlots][2]: what the pass kernels read of the states
-  void* outHost = nullptr;  // pinned copy of out
+  void* outHost = nullptr;  // pinned copies of out: two halves, by chunk parity
+  hipEvent_t polled[2] = {nullptr, nullptr};  // after each half's copy
   int outHostCap = 0;
   GraphCache graph;
   GraphCache burst;     // the timed burst of one pass kernel (DV_PARAM_PROFILE)
