@@ -1353,16 +1353,18 @@ Tables tablesOf(Engine& E, TileSet& set) {
   return T;
 }
 
-// Grid of the contact kernels: one 16-wavefront workgroup per CU (two with
-// several slots), never more than the tiles a single slot can hold, so that
-// the captured graph survives until a capacity grows.
+// Grid of the contact kernels.  A single slot gets one workgroup per row tile
+// its contact capacity can hold (the capacity, not the count, so that the
+// captured graph survives until a capacity grows): a workgroup that had to
+// take a second tile would double the kernel's latency, as at 257 tiles on
+// 256 CUs.  Several slots share two workgroups per CU over their tiles.
 template <typename TM>
 void grids(Engine& E, const TileSet& set, int& gc, int& gb) {
   if (E.nSlots == 1) {
     const Args<TM>& a = slotArgs<TM>(E, 0);
     const long itemsCap = a.cap + a.dims.nbil;
     const long tilesCap = std::max<long>(1, (itemsCap + kTC - 1) / kTC);
-    gc = (int)std::min<long>(tilesCap, E.numCU);
+    gc = (int)std::min<long>(tilesCap, 4L * E.numCU);
     gb = (int)std::max<long>(1, std::min<long>((a.dims.nb + kTB - 1) / kTB, 4L * E.numCU));
   } else {
     gc = 2 * E.numCU;
