@@ -96,7 +96,7 @@ struct SolveDims {
 
 namespace {
 
-constexpr int kTC = 1024;  // contact kernels: 16 wavefronts, one tile of rows
+constexpr int kTC = 512;   // contact kernels: 8 wavefronts, one tile of rows
 constexpr int kTB = 256;   // body kernel: one tile of bodies
 constexpr int kTD = 64;    // decide kernels: one wavefront per scene
 constexpr int kBacktrackCap = 1100;  // L doubles past 1e308 long before this
@@ -168,6 +168,7 @@ struct Args {
   TM* delta;            // mixed solver: gamma_new - y
   double* part1[2];     // kSlots per tile, by pass parity
   double* part3[2];
+  double* p1sum[2];     // single slot: P1's slots 0..5 totalled by P2's last workgroup
 };
 
 // Launch-wide tables (device memory, so that a captured graph stays valid
@@ -279,43 +280,70 @@ __device__ void reduceAll(const double* part1, const double* part3, int ng, doub
   for (int s = 0; s < 2 * kSlots; ++s) out[s] = (s % kSlots == kSlots - 1) ? waveMax(v[s]) : waveSum(v[s]);
 }
 
-// reduceAll for the pass decision, which needs only P1's slots 0..5 and P3's
-// slots 6 (sum) and 7 (max): eight values per tile instead of sixteen, so a
-// wavefront requests 512 tiles' worth at once.  Per slot the order of the
-// additions is reduceAll's, so the totals are the same; the other slots of
-// out are zero.
-__device__ void reducePass(const double* part1, const double* part3, int ng, double* out) {
-  constexpr int K = 8;  // tiles per lane per round
-  const int lane = threadIdx.x & 63;
-  double v[8];
+// Fixed-order totals over a slot's per-tile partials of NS consecutive slots
+// from s0 (the last slot, kSlots - 1, is a max, the others sums), by a
+// workgroup of kTP threads: thread t adds tiles t, t + kTP, ... in order (the
+// loads of eight tiles issued together), each wavefront combines its lanes by
+// a butterfly, and the wavefront totals are added in wavefront order.  out[]
+// holds the NS totals in every thread.  The pass decision reads P1's slots
+// 0..5 and P3's slots 6 and 7 this way, wherever it takes them from.
+constexpr int kTP = 256;
+template <int NS>
+__device__ void reduceSlots(const double* part, int s0, int ng, double (&out)[NS]) {
+  constexpr int K = 8, W = kTP / 64;
+  __shared__ double sh[W][NS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double v[NS];
 #pragma unroll
-  for (int s = 0; s < 8; ++s) v[s] = 0.0;
-  for (int base = 0; base < ng; base += 64 * K) {
-    double x[K][8];
+  for (int s = 0; s < NS; ++s) v[s] = 0.0;
+  for (int base = 0; base < ng; base += kTP * K) {
+    double x[K][NS];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int w = base + lane + 64 * k;
-      const int ws = w < ng ? w : 0;
+      const int w = base + (int)threadIdx.x + kTP * k;
+      const bool on = w < ng;
 #pragma unroll
-      for (int s = 0; s < 6; ++s) x[k][s] = gl(part1)[(long)ws * kSlots + s];
-      x[k][6] = gl(part3)[(long)ws * kSlots + 6];
-      x[k][7] = gl(part3)[(long)ws * kSlots + 7];
+      for (int s = 0; s < NS; ++s) x[k][s] = on ? gl(part)[(long)w * kSlots + s0 + s] : 0.0;
     }
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
+    for (int s = 0; s < NS; ++s) {
+      const bool isMax = s0 + s == kSlots - 1;
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const double xv = (base + lane + 64 * k) < ng ? x[k][s] : 0.0;
-        v[s] = s == 7 ? fmax(v[s], xv) : v[s] + xv;
-      }
+      for (int k = 0; k < K; ++k) v[s] = isMax ? fmax(v[s], x[k][s]) : v[s] + x[k][s];
     }
   }
 #pragma unroll
-  for (int s = 0; s < 2 * kSlots; ++s) out[s] = 0.0;
+  for (int s = 0; s < NS; ++s) {
+    const double t = s0 + s == kSlots - 1 ? waveMax(v[s]) : waveSum(v[s]);
+    if (lane == 0) sh[wave][s] = t;
+  }
+  __syncthreads();
 #pragma unroll
-  for (int s = 0; s < 6; ++s) out[s] = waveSum(v[s]);
-  out[kSlots + 6] = waveSum(v[6]);
-  out[kSlots + 7] = waveMax(v[7]);
+  for (int s = 0; s < NS; ++s) {
+    double t = sh[0][s];
+    for (int w = 1; w < W; ++w) t = s0 + s == kSlots - 1 ? fmax(t, sh[w][s]) : t + sh[w][s];
+    out[s] = t;
+  }
+  __syncthreads();
+}
+
+// The totals the pass decision needs: P1's slots 0..5 (from `p1`, already
+// reduced, when it is given) and P3's slots 6 and 7; the other slots are 0.
+__device__ void reducePass(const double* part1, const double* part3, int ng, const double* p1, double* out) {
+#pragma unroll
+  for (int s = 0; s < 2 * kSlots; ++s) out[s] = 0.0;
+  double t1[6], t3[2];
+  if (p1) {
+#pragma unroll
+    for (int s = 0; s < 6; ++s) t1[s] = gl(p1)[s];
+  } else {
+    reduceSlots<6>(part1, 0, ng, t1);
+  }
+  reduceSlots<2>(part3, 6, ng, t3);
+#pragma unroll
+  for (int s = 0; s < 6; ++s) out[s] = t1[s];
+  out[kSlots + 6] = t3[0];
+  out[kSlots + 7] = t3[1];
 }
 
 // ---------------------------------------------------------------------------
@@ -469,6 +497,20 @@ __device__ __forceinline__ TM bodyMinv(const Args<TM>& a, int g, int which, bool
 
 template <typename TM, int which>
 __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tables T, int sIdx) {
+  static_assert(kTB == kTP, "P2's extra workgroup reduces P1's partials like the decision");
+  // the body workgroups (a single slot's pass has one more, the last)
+  const int nw = (which == 2 && T.single) ? (int)gridDim.x - 1 : (int)gridDim.x;
+  if ((int)blockIdx.x == nw) {
+    // the pass's P1 partials (parity sIdx ^ 1), totalled while the other
+    // workgroups gather, so that the decision reads six numbers instead of
+    // every tile's (after the solve has ended the totals are simply not used)
+    const Args<TM>& a = A[0];
+    const int par = sIdx ^ 1;
+    double t1[6];
+    reduceSlots<6>(a.part1[par], 0, a.nTiles, t1);
+    if (threadIdx.x < 6) gl(a.p1sum[par])[threadIdx.x] = t1[threadIdx.x];
+    return;
+  }
   auto tileBody = [&](const Tile tile, const PassView& S, const Args<TM>& a) {
     // no lane leaves early: lanes past the last body read body 0's entries
     // (unused), so that the index loads and M^-1 are all requested at once
@@ -520,16 +562,16 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
     const Args<TM>& a = A[0];
     const int nt = T.counts[1];
     if (!(which == 2 && viewDone(S)))
-      for (int ti = blockIdx.x; ti < nt; ti += gridDim.x) tileBody(Tile{0, ti * kTB}, S, a);
+      for (int ti = blockIdx.x; ti < nt; ti += nw) tileBody(Tile{0, ti * kTB}, S, a);
   } else {
-    for (int ti = blockIdx.x, nt = T.counts[1]; ti < nt; ti += gridDim.x) {
+    for (int ti = blockIdx.x, nt = T.counts[1]; ti < nt; ti += nw) {
       const Tile tile = bodyTile(T, ti);
       const PassView S = loadView(T, tile.scene, sIdx);
       if (which == 2 && viewDone(S)) continue;
       tileBody(tile, S, A[tile.scene]);
     }
   }
-  for (int hq = blockIdx.x, nh = T.counts[2]; hq < nh; hq += gridDim.x) {
+  for (int hq = blockIdx.x, nh = T.counts[2]; hq < nh; hq += nw) {
     const int2 ref = T.heavy[hq];
     const PassView S = loadView(T, ref.x, sIdx);
     if (which == 2 && viewDone(S)) continue;
@@ -962,7 +1004,7 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
 // state st[par ^ 1] -> st[par], recorded in T.out for the host.  A slot that
 // is done copies its state forward.
 template <typename TM>
-__global__ __launch_bounds__(kTD) void kDecide(const Args<TM>* __restrict__ A, Tables T, int par) {
+__global__ __launch_bounds__(kTP) void kDecide(const Args<TM>* __restrict__ A, Tables T, int par) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   if (T.single) {
     // one slot, always active: the table entry and the state are requested
@@ -971,7 +1013,7 @@ __global__ __launch_bounds__(kTD) void kDecide(const Args<TM>* __restrict__ A, T
     const Args<TM>& a = A[0];
     ApgdState S = STATE(T, 0, par ^ 1);
     double tot[2 * kSlots];
-    reducePass(a.part1[par], a.part3[par], a.nTiles, tot);
+    reducePass(a.part1[par], a.part3[par], a.nTiles, a.p1sum[par], tot);
     if (!S.done) decideStep<kMixed>(S, tot, a.dims);
     if (threadIdx.x == 0) {
       STATE(T, 0, par) = S;
@@ -985,7 +1027,7 @@ __global__ __launch_bounds__(kTD) void kDecide(const Args<TM>* __restrict__ A, T
   ApgdState S = STATE(T, blockIdx.x, par ^ 1);
   if (!S.done) {
     double tot[2 * kSlots];
-    reducePass(a.part1[par], a.part3[par], a.nTiles, tot);
+    reducePass(a.part1[par], a.part3[par], a.nTiles, nullptr, tot);
     decideStep<kMixed>(S, tot, a.dims);
   }
   if (threadIdx.x == 0) {
@@ -1128,9 +1170,11 @@ constexpr int kProfBurst = 16;  // timed back-to-back launches of the profiled k
 template <typename TM>
 void launchPass(const Plan<TM>& P, hipStream_t st, int kern, int par) {
   if (kern == 1) launchP1<TM>(P, st, par);
-  if (kern == 2) hipLaunchKernelGGL((kBody<TM, 2>), dim3(P.gb), dim3(kTB), 0, st, P.A, P.T, par ^ 1);
+  // a single slot's P2 has one more workgroup, which totals P1's partials
+  if (kern == 2)
+    hipLaunchKernelGGL((kBody<TM, 2>), dim3(P.gb + (P.T.single ? 1 : 0)), dim3(kTB), 0, st, P.A, P.T, par ^ 1);
   if (kern == 3) hipLaunchKernelGGL((kP3<TM, 1>), dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
-  if (kern == 4) hipLaunchKernelGGL(kDecide<TM>, dim3(P.nSlots), dim3(kTD), 0, st, P.A, P.T, par);
+  if (kern == 4) hipLaunchKernelGGL(kDecide<TM>, dim3(P.nSlots), dim3(kTP), 0, st, P.A, P.T, par);
 }
 
 // kProfBurst launches of pass kernel `kern` (P1, P2 or P3) between ev[0] and
@@ -1241,7 +1285,7 @@ void prepareScene(Scene& s, Args<TM>& a) {
   const long RsCap = std::max<long>(3L * s.ncCap + s.nbil, 1);
   const long itemsCap = (long)s.ncCap + s.nbil;
   const long tilesCap = std::max<long>(1, (itemsCap + kTC - 1) / kTC);
-  s.partials.reserve((size_t)4 * kSlots * tilesCap);
+  s.partials.reserve((size_t)4 * kSlots * tilesCap + 2 * kSlots);
   s.work.reserve((size_t)5 * RsCap + 16);  // ring[4], gHat: FP64 in both solvers
   // TM block: rn[4], r, delta, then J (per contact) and tmp (per body) of 4
   // components 32-byte aligned, then (mixed) M^-1
@@ -1293,6 +1337,7 @@ void prepareScene(Scene& s, Args<TM>& a) {
   for (int k = 0; k < 2; ++k) {
     a.part1[k] = s.partials.p + (long)(2 * k) * kSlots * tilesCap;
     a.part3[k] = s.partials.p + (long)(2 * k + 1) * kSlots * tilesCap;
+    a.p1sum[k] = s.partials.p + 4L * kSlots * tilesCap + k * kSlots;
   }
 }
 
@@ -1354,17 +1399,19 @@ Tables tablesOf(Engine& E, TileSet& set) {
 }
 
 // Grid of the contact kernels.  A single slot gets one workgroup per row tile
-// its contact capacity can hold (the capacity, not the count, so that the
-// captured graph survives until a capacity grows): a workgroup that had to
-// take a second tile would double the kernel's latency, as at 257 tiles on
-// 256 CUs.  Several slots share two workgroups per CU over their tiles.
+// its contact capacity can hold, up to 16 per CU (the capacity, not the count,
+// so that the captured graph survives until a capacity grows).  Tiles are
+// 512 rows: with 1024-row tiles, 257 tiles on 256 CUs made one CU carry twice
+// the rows of the others, and P1 took 35% longer at 263k contacts than at
+// 262k; 256-row tiles balance better still but cost more in partial sums.
+// Several slots share two workgroups per CU over their tiles.
 template <typename TM>
 void grids(Engine& E, const TileSet& set, int& gc, int& gb) {
   if (E.nSlots == 1) {
     const Args<TM>& a = slotArgs<TM>(E, 0);
     const long itemsCap = a.cap + a.dims.nbil;
     const long tilesCap = std::max<long>(1, (itemsCap + kTC - 1) / kTC);
-    gc = (int)std::min<long>(tilesCap, 4L * E.numCU);
+    gc = (int)std::min<long>(tilesCap, 16L * E.numCU);
     gb = (int)std::max<long>(1, std::min<long>((a.dims.nb + kTB - 1) / kTB, 4L * E.numCU));
   } else {
     gc = 2 * E.numCU;
