@@ -68,6 +68,7 @@ struct ApgdState {
   int it, bt, backtracks, restart;
   int cur, prev, nxt;
   int hatSrc, done, converged, passes;
+  int pendingRes;  // the iterate in slot cur was accepted and its residual is still to be taken
 };
 
 // What the pass kernels read of a slot's state, written beside the state by
@@ -90,7 +91,7 @@ __device__ __forceinline__ PassView viewOf(const ApgdState& S) {
 
 struct SolveDims {
   int nc, nbil, nb, maxIt;
-  double mu, tol, gdiff;
+  double mu, tol, gdiff, igdiff;  // igdiff = 1 / gdiff (APGD.cu:98-100)
   int warm, geomMode;
 };
 
@@ -168,7 +169,7 @@ struct Args {
   TM* delta;            // mixed solver: gamma_new - y
   double* part1[2];     // kSlots per tile, by pass parity
   double* part3[2];
-  double* p1sum[2];     // single slot: P1's slots 0..5 totalled by P2's last workgroup
+  double* p1sum[2];     // single slot: P1's slots totalled by P2's last workgroup
 };
 
 // Launch-wide tables (device memory, so that a captured graph stays valid
@@ -286,7 +287,7 @@ __device__ void reduceAll(const double* part1, const double* part3, int ng, doub
 // loads of eight tiles issued together), each wavefront combines its lanes by
 // a butterfly, and the wavefront totals are added in wavefront order.  out[]
 // holds the NS totals in every thread.  The pass decision reads P1's slots
-// 0..5 and P3's slots 6 and 7 this way, wherever it takes them from.
+// and P3's slot 6 this way, wherever it takes them from.
 constexpr int kTP = 256;
 template <int NS>
 __device__ void reduceSlots(const double* part, int s0, int ng, double (&out)[NS]) {
@@ -327,23 +328,23 @@ __device__ void reduceSlots(const double* part, int s0, int ng, double (&out)[NS
   __syncthreads();
 }
 
-// The totals the pass decision needs: P1's slots 0..5 (from `p1`, already
-// reduced, when it is given) and P3's slots 6 and 7; the other slots are 0.
+// The totals the pass decision needs: P1's slots 0..5 and 7 (from `p1`,
+// already reduced, when it is given; slot 6 is unused) and P3's slot 6; the
+// other slots are 0.
 __device__ void reducePass(const double* part1, const double* part3, int ng, const double* p1, double* out) {
 #pragma unroll
   for (int s = 0; s < 2 * kSlots; ++s) out[s] = 0.0;
-  double t1[6], t3[2];
+  double t1[kSlots], t3[1];
   if (p1) {
 #pragma unroll
-    for (int s = 0; s < 6; ++s) t1[s] = gl(p1)[s];
+    for (int s = 0; s < kSlots; ++s) t1[s] = gl(p1)[s];
   } else {
-    reduceSlots<6>(part1, 0, ng, t1);
+    reduceSlots<kSlots>(part1, 0, ng, t1);
   }
-  reduceSlots<2>(part3, 6, ng, t3);
+  reduceSlots<1>(part3, 6, ng, t3);
 #pragma unroll
-  for (int s = 0; s < 6; ++s) out[s] = t1[s];
+  for (int s = 0; s < kSlots; ++s) out[s] = t1[s];
   out[kSlots + 6] = t3[0];
-  out[kSlots + 7] = t3[1];
 }
 
 // ---------------------------------------------------------------------------
@@ -402,10 +403,9 @@ __device__ __forceinline__ void contactRows(const Args<TM>& a, int c, const TM (
 }
 
 __device__ __forceinline__ double residualContact(double g0, double g1, double g2, double w0, double w1,
-                                                  double w2, double mu, double gdiff) {
+                                                  double w2, double mu, double gdiff, double ig) {
   double z0 = 1.0 * g0 + (-gdiff) * w0, z1 = 1.0 * g1 + (-gdiff) * w1, z2 = 1.0 * g2 + (-gdiff) * w2;
   projectCone<double>(z0, z1, z2, mu);
-  const double ig = 1.0 / gdiff;
   const double e0 = ig * g0 + (-ig) * z0, e1 = ig * g1 + (-ig) * z1, e2 = ig * g2 + (-ig) * z2;
   return fmax(fabs(e0), fmax(fabs(e1), fabs(e2)));
 }
@@ -502,13 +502,13 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
   const int nw = (which == 2 && T.single) ? (int)gridDim.x - 1 : (int)gridDim.x;
   if ((int)blockIdx.x == nw) {
     // the pass's P1 partials (parity sIdx ^ 1), totalled while the other
-    // workgroups gather, so that the decision reads six numbers instead of
+    // workgroups gather, so that the decision reads eight numbers instead of
     // every tile's (after the solve has ended the totals are simply not used)
     const Args<TM>& a = A[0];
     const int par = sIdx ^ 1;
-    double t1[6];
-    reduceSlots<6>(a.part1[par], 0, a.nTiles, t1);
-    if (threadIdx.x < 6) gl(a.p1sum[par])[threadIdx.x] = t1[threadIdx.x];
+    double t1[kSlots];
+    reduceSlots<kSlots>(a.part1[par], 0, a.nTiles, t1);
+    if (threadIdx.x < kSlots) gl(a.p1sum[par])[threadIdx.x] = t1[threadIdx.x];
     return;
   }
   auto tileBody = [&](const Tile tile, const PassView& S, const Args<TM>& a) {
@@ -691,6 +691,7 @@ __global__ __launch_bounds__(kTD) void kSetupDecide(const Args<TM>* __restrict__
     for (int k = 0; k < 2 * kSlots; ++k) S.sum[k] = 0;
     S.it = 0; S.bt = 0; S.backtracks = 0; S.restart = 1;
     S.cur = 0; S.prev = 0; S.nxt = 1; S.hatSrc = -1;  // -1: gHat (ones)
+    S.pendingRes = 0;  // gamma_0 is not an accepted iterate (APGD.cu:228-230)
     S.done = a.dims.maxIt <= 0 ? 1 : 0;
     S.converged = 0;
     S.passes = 0;
@@ -712,6 +713,28 @@ __device__ void decideStep(ApgdState& S, const double* tot, const SolveDims& D) 
   const double* s1 = tot;           // P1 slots
   const double* s3 = tot + kSlots;  // P3 slots
   S.passes++;
+  // getResidual of the iterate the previous decision accepted (now in slot
+  // cur), evaluated by this pass's P1, then gammaHat and the stopping test
+  // (APGD.cu:276-289).  Taking it one pass late spares P3 the iterate and r;
+  // the decisions and the result are those of the reference's order.
+  if (S.pendingRes) {
+    S.pendingRes = 0;
+    const double res = s1[kSlots - 1];
+    if (res < S.best) {  // gammaHat := that iterate, kept in its ring slot
+      S.best = res;
+      S.hatSrc = S.cur;
+    }
+    if (S.best < D.tol) {
+      S.done = 1;
+      S.converged = 1;
+      S.it -= 1;  // the reference stops inside the iteration it counts
+      return;
+    }
+    if (S.it >= D.maxIt) {
+      S.done = 1;
+      return;
+    }
+  }
   const double L = S.L;
   bool backtrack;
   if (kMixed) {
@@ -733,16 +756,6 @@ __device__ void decideStep(ApgdState& S, const double* tot, const SolveDims& D) 
   const double theta = S.theta;
   double thetaNew = (-(theta * theta) + theta * sqrt(theta * theta + 4.0)) / 2.0;
   S.beta = theta * (1.0 - theta) / (theta * theta + thetaNew);
-  const double res = s3[7];
-  if (res < S.best) {  // gammaHat := gamma_new, kept in its ring slot
-    S.best = res;
-    S.hatSrc = S.nxt;
-  }
-  if (S.best < D.tol) {
-    S.done = 1;
-    S.converged = 1;
-    return;
-  }
   if (s1[5] > 0) {
     S.restart = 1;
     thetaNew = 1.0;
@@ -759,7 +772,7 @@ __device__ void decideStep(ApgdState& S, const double* tot, const SolveDims& D) 
     if (k != S.cur && k != S.prev && k != S.hatSrc) { S.nxt = k; break; }
   S.it++;
   S.bt = 0;
-  if (S.it >= D.maxIt) S.done = 1;
+  S.pendingRes = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -851,6 +864,27 @@ __global__ __launch_bounds__(kTC / R) void kP1(const Args<TM>* __restrict__ A, T
           gn[rr] = 1.0 * y[rr] + (-t) * gg[rr];
         }
       }
+      // getResidual (APGD.cu:90-102) of gamma_k, with N gamma_k and r
+      // already loaded: gamma_k is the iterate the previous decision
+      // accepted, whose residual that decision left to this pass
+      {
+        const double w0 = (double)nk[u][0] + (double)rv[u][0];
+        if (bil) {
+          double e;
+          if (D.nc) {
+            const double z = 1.0 * gk[u][0] + (-D.gdiff) * w0;
+            const double ig = D.igdiff;
+            e = fabs(ig * gk[u][0] + (-ig) * z);
+          } else {
+            e = fabs(w0);
+          }
+          acc[7] = fmax(acc[7], e);
+        } else {
+          const double w1 = (double)nk[u][1] + (double)rv[u][1];
+          const double w2 = (double)nk[u][2] + (double)rv[u][2];
+          acc[7] = fmax(acc[7], residualContact(gk[u][0], gk[u][1], gk[u][2], w0, w1, w2, D.mu, D.gdiff, D.igdiff));
+        }
+      }
       if (!bil) projectCone<double>(gn[0], gn[1], gn[2], D.mu);
 #pragma unroll
       for (int rr = 0; rr < 3; ++rr) {
@@ -911,7 +945,7 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
     // return in order, so the tmp gathers, which need the ids, wait only for
     // them and go out while the rest are still in flight
     double g[R][3];
-    TM dl[R][3], rv[R][3], fc[R][6];
+    TM dl[R][3], fc[R][6];
     int ca[R], cb[R];
 #pragma unroll
     for (int u = 0; u < R; ++u) {
@@ -928,8 +962,7 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
 #pragma unroll
       for (int rr = 0; rr < 3; ++rr) {
         const long f = ROW(a, rr, qc);
-        g[u][rr] = Gn[f];
-        rv[u][rr] = gl(a.r)[f];
+        g[u][rr] = kMixed ? 0.0 : Gn[f];  // the FP64 test needs gamma_new' N gamma_new
         dl[u][rr] = kMixed ? gl(a.delta)[f] : (TM)0;
       }
 #pragma unroll
@@ -951,18 +984,7 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
         const long f = BROW(a, q - D.nc);
         const TM nv = bilRow(a, (int)(q - D.nc));
         Nn[f] = nv;
-        const double gv = Gn[f];
-        acc[6] += (kMixed ? (double)gl(a.delta)[f] : gv) * (double)nv;
-        const double w = (double)nv + (double)gl(a.r)[f];
-        double e;
-        if (D.nc) {
-          const double z = 1.0 * gv + (-D.gdiff) * w;
-          const double ig = 1.0 / D.gdiff;
-          e = fabs(ig * gv + (-ig) * z);
-        } else {
-          e = fabs(w);
-        }
-        acc[7] = fmax(acc[7], e);
+        acc[6] += (kMixed ? (double)gl(a.delta)[f] : Gn[f]) * (double)nv;
       } else {
         TM F[9];
         frameFrom(fc[u], D.geomMode, F);
@@ -971,15 +993,10 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
         const TM n1 = F[3] * d0 + F[4] * d1 + F[5] * d2;
         const TM n2 = F[6] * d0 + F[7] * d1 + F[8] * d2;
         Nn[ROW(a, 0, q)] = n0; Nn[ROW(a, 1, q)] = n1; Nn[ROW(a, 2, q)] = n2;
-        const double g0 = g[u][0], g1 = g[u][1], g2 = g[u][2];
         if (kMixed)
           acc[6] += (double)dl[u][0] * (double)n0 + (double)dl[u][1] * (double)n1 + (double)dl[u][2] * (double)n2;
         else
-          acc[6] += g0 * (double)n0 + g1 * (double)n1 + g2 * (double)n2;
-        const double w0 = (double)n0 + (double)rv[u][0];
-        const double w1 = (double)n1 + (double)rv[u][1];
-        const double w2 = (double)n2 + (double)rv[u][2];
-        acc[7] = fmax(acc[7], residualContact(g0, g1, g2, w0, w1, w2, D.mu, D.gdiff));
+          acc[6] += g[u][0] * (double)n0 + g[u][1] * (double)n1 + g[u][2] * (double)n2;
       }
     }
     writePartials<NT>(acc, TILE_PART(a.part3[par], tile));
@@ -1304,6 +1321,7 @@ void prepareScene(Scene& s, Args<TM>& a) {
   const double cnt = (double)((long)s.nc + s.nbil);
   a.dims.nc = s.nc; a.dims.nbil = s.nbil; a.dims.nb = s.nb; a.dims.maxIt = s.maxIt;
   a.dims.mu = s.mu; a.dims.tol = s.tol; a.dims.gdiff = 1.0 / (cnt * cnt);
+  a.dims.igdiff = 1.0 / a.dims.gdiff;
   a.dims.warm = s.warm ? 1 : 0; a.dims.geomMode = s.geomMode;
   const long items = (long)s.nc + s.nbil;
   a.nTiles = (int)((items + kTC - 1) / kTC);
