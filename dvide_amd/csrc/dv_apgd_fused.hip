@@ -800,9 +800,18 @@ __device__ __forceinline__ void frameFrom(const float (&c)[6], int, float (&F)[9
 // kTC / R threads covers one tile of kTC rows, and every thread issues the
 // loads of all its rows before computing, for more memory-level parallelism
 // per wavefront.  The per-tile partial sums keep their layout.
+//
+// P1 with one row per thread is held to 80 VGPRs (six wavefronts per SIMD;
+// the FP64 instance spills a few registers to scratch), so that a CU keeps three
+// of its 512-thread workgroups in flight rather than two: past two tiles per
+// CU (262,144 rows on 256 CUs), the CUs given a third tile then run it beside
+// the other two instead of after one of them (P1 11.3 us instead of 12.7 us
+// at 263k contacts, 10.4 us against 10.2 us at 256k).
+template <int R>
+constexpr int kP1MinWaves = R == 1 ? 6 : 1;
 
 template <typename TM, int R>
-__global__ __launch_bounds__(kTC / R) void kP1(const Args<TM>* __restrict__ A, Tables T, int par) {
+__global__ __launch_bounds__(kTC / R, kP1MinWaves<R>) void kP1(const Args<TM>* __restrict__ A, Tables T, int par) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   constexpr int NT = kTC / R;
   constexpr int NF = kMixed ? 6 : 3;  // stored frame components
@@ -915,12 +924,15 @@ __global__ __launch_bounds__(kTC / R) void kP1(const Args<TM>* __restrict__ A, T
   };
   if (T.single) {
     // one slot: its view, table entry and tile count do not depend on the
-    // tile, so they are requested together before the tile loop
+    // tile, so they are requested together before the tile.  The grid has a
+    // workgroup for every tile the slot's capacity can hold, so each takes
+    // at most one tile and nothing stays live across a tile loop (which held
+    // P1 above 80 VGPRs).
     const PassView S = loadView(T, 0, par ^ 1);
     const Args<TM>& a = A[0];
     const int nt = T.counts[0];
     if (viewDone(S)) return;
-    for (int ti = blockIdx.x; ti < nt; ti += gridDim.x) tileBody(Tile{0, ti * kTC}, S, a);
+    if ((int)blockIdx.x < nt) tileBody(Tile{0, (int)blockIdx.x * kTC}, S, a);
     return;
   }
   TILE_LOOP(T) {
@@ -1417,19 +1429,21 @@ Tables tablesOf(Engine& E, TileSet& set) {
 }
 
 // Grid of the contact kernels.  A single slot gets one workgroup per row tile
-// its contact capacity can hold, up to 16 per CU (the capacity, not the count,
-// so that the captured graph survives until a capacity grows).  Tiles are
-// 512 rows: with 1024-row tiles, 257 tiles on 256 CUs made one CU carry twice
-// the rows of the others, and P1 took 35% longer at 263k contacts than at
-// 262k; 256-row tiles balance better still but cost more in partial sums.
-// Several slots share two workgroups per CU over their tiles.
+// its contact capacity can hold (the capacity, not the count, so that the
+// captured graph survives until a capacity grows; P1 relies on this grid
+// covering every tile).  Tiles are 512 rows: with 1024-row tiles, 257 tiles
+// on 256 CUs made one CU carry twice the rows of the others, and P1 took 35%
+// longer at 263k contacts than at 262k; 256-row tiles balance better still
+// but cost more in partial sums.  Past 512 tiles a CU holds three P1
+// workgroups at once (kP1MinWaves).  Several slots share two workgroups per
+// CU over their tiles.
 template <typename TM>
 void grids(Engine& E, const TileSet& set, int& gc, int& gb) {
   if (E.nSlots == 1) {
     const Args<TM>& a = slotArgs<TM>(E, 0);
     const long itemsCap = a.cap + a.dims.nbil;
     const long tilesCap = std::max<long>(1, (itemsCap + kTC - 1) / kTC);
-    gc = (int)std::min<long>(tilesCap, 16L * E.numCU);
+    gc = (int)tilesCap;
     gb = (int)std::max<long>(1, std::min<long>((a.dims.nb + kTB - 1) / kTB, 4L * E.numCU));
   } else {
     gc = 2 * E.numCU;
