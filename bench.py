@@ -418,7 +418,7 @@ def main():
         "phases_ms_last_step": {k: round(v, 3) for k, v in mixed["phases"].items()},
         "roofline": {
             "bound": "hbm",
-            "kernel": "kP1<float> (contact pass: momentum, projection, F^T gamma)",
+            "kernel": "kP1<float> (contact pass: previous decision, momentum, projection, F^T gamma)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

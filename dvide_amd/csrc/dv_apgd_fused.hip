@@ -287,13 +287,15 @@ __device__ void reduceAll(const double* part1, const double* part3, int ng, doub
 // loads of eight tiles issued together), each wavefront combines its lanes by
 // a butterfly, and the wavefront totals are added in wavefront order.  out[]
 // holds the NS totals in every thread.  The pass decision reads P1's slots
-// and P3's slot 6 this way, wherever it takes them from.
+// and P3's slot 6 this way, wherever it takes them from.  In a wider
+// workgroup the threads past kTP add nothing, so the totals are the same.
 constexpr int kTP = 256;
 template <int NS>
 __device__ void reduceSlots(const double* part, int s0, int ng, double (&out)[NS]) {
   constexpr int K = 8, W = kTP / 64;
   __shared__ double sh[W][NS];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool mine = threadIdx.x < kTP;
   double v[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) v[s] = 0.0;
@@ -302,7 +304,7 @@ __device__ void reduceSlots(const double* part, int s0, int ng, double (&out)[NS
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int w = base + (int)threadIdx.x + kTP * k;
-      const bool on = w < ng;
+      const bool on = mine && w < ng;
 #pragma unroll
       for (int s = 0; s < NS; ++s) x[k][s] = on ? gl(part)[(long)w * kSlots + s0 + s] : 0.0;
     }
@@ -316,7 +318,7 @@ __device__ void reduceSlots(const double* part, int s0, int ng, double (&out)[NS
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const double t = s0 + s == kSlots - 1 ? waveMax(v[s]) : waveSum(v[s]);
-    if (lane == 0) sh[wave][s] = t;
+    if (lane == 0 && wave < W) sh[wave][s] = t;
   }
   __syncthreads();
 #pragma unroll
@@ -328,22 +330,28 @@ __device__ void reduceSlots(const double* part, int s0, int ng, double (&out)[NS
   __syncthreads();
 }
 
-// The totals the pass decision needs: P1's slots 0..5 and 7 (from `p1`,
-// already reduced, when it is given; slot 6 is unused) and P3's slot 6; the
-// other slots are 0.
-__device__ void reducePass(const double* part1, const double* part3, int ng, const double* p1, double* out) {
+// The totals the pass decision needs: P1's slots 0..5 and 7 (slot 6 is
+// unused) and P3's slot 6; the other slots are 0.  reducePass reduces P1's
+// partials; passTotals takes them already totalled (a single slot's P2
+// totals them), which keeps the registers of the eight-slot reduction out of
+// the kernels that decide a single slot.
+__device__ void reducePass(const double* part1, const double* part3, int ng, double* out) {
 #pragma unroll
   for (int s = 0; s < 2 * kSlots; ++s) out[s] = 0.0;
   double t1[kSlots], t3[1];
-  if (p1) {
-#pragma unroll
-    for (int s = 0; s < kSlots; ++s) t1[s] = gl(p1)[s];
-  } else {
-    reduceSlots<kSlots>(part1, 0, ng, t1);
-  }
+  reduceSlots<kSlots>(part1, 0, ng, t1);
   reduceSlots<1>(part3, 6, ng, t3);
 #pragma unroll
   for (int s = 0; s < kSlots; ++s) out[s] = t1[s];
+  out[kSlots + 6] = t3[0];
+}
+__device__ void passTotals(const double* p1, const double* part3, int ng, double* out) {
+#pragma unroll
+  for (int s = 0; s < 2 * kSlots; ++s) out[s] = 0.0;
+  double t3[1];
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) out[s] = gl(p1)[s];
+  reduceSlots<1>(part3, 6, ng, t3);
   out[kSlots + 6] = t3[0];
 }
 
@@ -810,7 +818,30 @@ __device__ __forceinline__ void frameFrom(const float (&c)[6], int, float (&F)[9
 template <int R>
 constexpr int kP1MinWaves = R == 1 ? 6 : 1;
 
-template <typename TM, int R>
+// The decision of the previous pass (parity par ^ 1) of a single slot, taken
+// by every workgroup of the next pass's P1 instead of by a kernel of its own:
+// the same state, totals and summation order as kDecide give the same new
+// state, which workgroup 0 stores, and the pass view follows from it without
+// a reload.  P1 reads state par and writes state par ^ 1 and only its own
+// partial sums, so a repeated launch reproduces the same results.
+template <typename TM>
+__device__ PassView decidePrevious(const Args<TM>& a, const Tables& T, int par) {
+  constexpr bool kMixed = sizeof(TM) < sizeof(double);
+  ApgdState S = STATE(T, 0, par);
+  double tot[2 * kSlots];
+  passTotals(a.p1sum[par ^ 1], a.part3[par ^ 1], a.nTiles, tot);
+  if (!S.done) decideStep<kMixed>(S, tot, a.dims);
+  const PassView v = viewOf(S);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    STATE(T, 0, par ^ 1) = S;
+    VIEW(T, 0, par ^ 1) = v;
+    T.out[0] = S;
+  }
+  return v;
+}
+
+// kFold: a single slot's P1 takes the previous pass's decision first.
+template <typename TM, int R, bool kFold>
 __global__ __launch_bounds__(kTC / R, kP1MinWaves<R>) void kP1(const Args<TM>* __restrict__ A, Tables T, int par) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   constexpr int NT = kTC / R;
@@ -928,9 +959,15 @@ __global__ __launch_bounds__(kTC / R, kP1MinWaves<R>) void kP1(const Args<TM>* _
     // workgroup for every tile the slot's capacity can hold, so each takes
     // at most one tile and nothing stays live across a tile loop (which held
     // P1 above 80 VGPRs).
-    const PassView S = loadView(T, 0, par ^ 1);
     const Args<TM>& a = A[0];
     const int nt = T.counts[0];
+    PassView S;
+    if (kFold) {
+      if (blockIdx.x != 0 && (int)blockIdx.x >= nt) return;  // workgroup 0 stores the state
+      S = decidePrevious(a, T, par);
+    } else {
+      S = loadView(T, 0, par ^ 1);
+    }
     if (viewDone(S)) return;
     if ((int)blockIdx.x < nt) tileBody(Tile{0, (int)blockIdx.x * kTC}, S, a);
     return;
@@ -1042,7 +1079,7 @@ __global__ __launch_bounds__(kTP) void kDecide(const Args<TM>* __restrict__ A, T
     const Args<TM>& a = A[0];
     ApgdState S = STATE(T, 0, par ^ 1);
     double tot[2 * kSlots];
-    reducePass(a.part1[par], a.part3[par], a.nTiles, a.p1sum[par], tot);
+    passTotals(a.p1sum[par], a.part3[par], a.nTiles, tot);
     if (!S.done) decideStep<kMixed>(S, tot, a.dims);
     if (threadIdx.x == 0) {
       STATE(T, 0, par) = S;
@@ -1056,7 +1093,7 @@ __global__ __launch_bounds__(kTP) void kDecide(const Args<TM>* __restrict__ A, T
   ApgdState S = STATE(T, blockIdx.x, par ^ 1);
   if (!S.done) {
     double tot[2 * kSlots];
-    reducePass(a.part1[par], a.part3[par], a.nTiles, nullptr, tot);
+    reducePass(a.part1[par], a.part3[par], a.nTiles, tot);
     decideStep<kMixed>(S, tot, a.dims);
   }
   if (threadIdx.x == 0) {
@@ -1184,9 +1221,11 @@ struct Plan {
 };
 
 template <typename TM>
-void launchP1(const Plan<TM>& P, hipStream_t st, int par) {
-  if (P.rpt == 2) hipLaunchKernelGGL((kP1<TM, 2>), dim3(P.gc), dim3(kTC / 2), 0, st, P.A, P.T, par);
-  else hipLaunchKernelGGL((kP1<TM, 1>), dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
+void launchP1(const Plan<TM>& P, hipStream_t st, int par, bool fold) {
+  if (P.rpt == 2 && fold) hipLaunchKernelGGL((kP1<TM, 2, true>), dim3(P.gc), dim3(kTC / 2), 0, st, P.A, P.T, par);
+  else if (P.rpt == 2) hipLaunchKernelGGL((kP1<TM, 2, false>), dim3(P.gc), dim3(kTC / 2), 0, st, P.A, P.T, par);
+  else if (fold) hipLaunchKernelGGL((kP1<TM, 1, true>), dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
+  else hipLaunchKernelGGL((kP1<TM, 1, false>), dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
 }
 template <typename TM>
 void launchP3(const Plan<TM>& P, hipStream_t st, int par) {
@@ -1197,8 +1236,8 @@ void launchP3(const Plan<TM>& P, hipStream_t st, int par) {
 constexpr int kProfBurst = 16;  // timed back-to-back launches of the profiled kernel
 
 template <typename TM>
-void launchPass(const Plan<TM>& P, hipStream_t st, int kern, int par) {
-  if (kern == 1) launchP1<TM>(P, st, par);
+void launchPass(const Plan<TM>& P, hipStream_t st, int kern, int par, bool fold) {
+  if (kern == 1) launchP1<TM>(P, st, par, fold);
   // a single slot's P2 has one more workgroup, which totals P1's partials
   if (kern == 2)
     hipLaunchKernelGGL((kBody<TM, 2>), dim3(P.gb + (P.T.single ? 1 : 0)), dim3(kTB), 0, st, P.A, P.T, par ^ 1);
@@ -1211,20 +1250,22 @@ void launchPass(const Plan<TM>& P, hipStream_t st, int kern, int par) {
 // is the kernel's duration in a graph rather than that plus a stream
 // launch's latency.  The graph is kept for the next solve of the same plan.
 template <typename TM>
-void timedBurst(const Plan<TM>& P, hipStream_t st, int kern, int par, hipEvent_t* ev, GraphCache& cache) {
+void timedBurst(const Plan<TM>& P, hipStream_t st, int kern, int par, bool fold, hipEvent_t* ev,
+                GraphCache& cache) {
   struct Key {
     Plan<TM> P;
-    int kern, par;
+    int kern, par, fold;
   } key;
   std::memset(&key, 0, sizeof(key));
   key.P = P;
   key.kern = kern;
   key.par = par;
+  key.fold = fold ? 1 : 0;
   if (!cache.matches(&key, sizeof(key))) {
     cache.reset();
     hipGraph_t graph;
     DV_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int r = 0; r < kProfBurst; ++r) launchPass<TM>(P, st, kern, par);
+    for (int r = 0; r < kProfBurst; ++r) launchPass<TM>(P, st, kern, par, fold);
     DV_HIP(hipStreamEndCapture(st, &graph));
     hipError_t e = hipGraphInstantiate(&cache.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -1237,23 +1278,32 @@ void timedBurst(const Plan<TM>& P, hipStream_t st, int kern, int par, hipEvent_t
   DV_HIP(hipEventRecord(ev[1], st));
 }
 
-// One chunk of n (even) passes, pass k with parity k & 1.  With `ev` (stream
-// launches only: events recorded inside a captured graph cannot be timed),
-// kernel `prof` of the chunk's first pass runs as a timed burst (P1, P2 and
-// P3 read nothing they write, so the repeats reproduce the same outputs);
-// the decide kernel, which is not repeatable, is timed alone.
+// One chunk of n (even) passes, pass k with parity k & 1.  A single slot's
+// decision is taken by the next pass's P1, so its chunk launches the decide
+// kernel once, after the last pass (the host reads the states after a
+// chunk), and the first pass's P1 reads the view that kernel left.  With `ev`
+// (stream launches only: events recorded inside a captured graph cannot be
+// timed), kernel `prof` runs as a timed burst (P1, P2 and P3 read nothing
+// they write, so the repeats reproduce the same outputs) in the chunk's first
+// pass, or for a P1 that decides, its second (whose decision has this
+// chunk's partial sums); the decide kernel, which is not repeatable, is
+// timed alone where it runs.
 template <typename TM>
 void enqueuePasses(const Plan<TM>& P, hipStream_t st, int n, hipEvent_t* ev, int prof, GraphCache* burst) {
+  const bool single = P.T.single != 0;
   for (int k = 0; k < n; ++k) {
     const int par = k & 1;
+    const bool fold = single && k > 0;
     for (int kern = 1; kern <= 4; ++kern) {
-      const bool timed = ev && prof == kern && k == 0;
+      if (kern == 4 && single && k != n - 1) continue;
+      const int kt = !single ? 0 : kern == 1 ? 1 : kern == 4 ? n - 1 : 0;
+      const bool timed = ev && prof == kern && k == kt;
       if (timed && kern != 4) {
-        timedBurst<TM>(P, st, kern, par, ev, *burst);
+        timedBurst<TM>(P, st, kern, par, fold, ev, *burst);
         continue;
       }
       if (timed) DV_HIP(hipEventRecord(ev[0], st));
-      launchPass<TM>(P, st, kern, par);
+      launchPass<TM>(P, st, kern, par, fold);
       if (timed) DV_HIP(hipEventRecord(ev[1], st));
     }
   }

@@ -65,14 +65,17 @@ typedef struct dv_scene dv_scene;
 #define DV_PARAM_VERBOSE 12   /* Solver::verbose */
 #define DV_PARAM_GRID 13      /* unused (accepted for compatibility) */
 #define DV_PARAM_PROFILE 14   /* time one kernel of the graph-driven solvers with HIP
-                                 events: 0 off, 1 P1 (contacts), 2 P2 (bodies),
+                                 events: 0 off, 1 P1 (contacts; for one scene it also
+                                 takes the previous pass's decision), 2 P2 (bodies),
                                  3 P3 (contacts), 4 the decide kernel.  One chunk of 16
                                  passes per solve (the first after the first chunk whose
                                  entry state has the momentum term on) is then launched
-                                 on the scene's stream, and in its first pass that
-                                 kernel runs as a captured graph of 16 back-to-back
-                                 launches between two events (the decide kernel once,
-                                 alone); the rest run in the captured chunk graph */
+                                 on the scene's stream, and in its first pass (a deciding
+                                 P1: its second) that kernel runs as a captured graph of
+                                 16 back-to-back launches between two events (the decide
+                                 kernel once, alone, where it runs: after the chunk's
+                                 last pass for one scene); the rest run in the captured
+                                 chunk graph */
 #define DV_PARAM_ROWS_PER_THREAD 15 /* contact kernels of the graph-driven solvers: 1 or 2
                                        contacts per thread (a tuning choice; results agree
                                        to rounding, each choice is deterministic) */
