@@ -7,18 +7,19 @@
 // three kernels over all active scenes:
 //   P1 (contacts): y and N y from the last two iterates, g = N y + r,
 //       gamma_new = Pi(y - t g), the dot products of the objective and
-//       restart tests, and J_c = F_c^T gamma_new (the world-frame impulse of
-//       contact c on body A);
+//       restart tests, J_c = F_c^T gamma_new (the world-frame impulse of
+//       contact c on body A), and the infinity norm of getResidual
+//       (APGD.cu:90-102) of the iterate the previous decision accepted;
 //   P2 (bodies):   tmp_b = M_b^-1 (sum_A J_c - sum_B J_c + bilateral rows);
 //       bodies with many contacts are reduced by a whole workgroup;
-//   P3 (contacts): (N gamma_new)_c = F_c (tmp_A - tmp_B), gamma_new . N gamma_new
-//       and the residual's infinity norm (getResidual, APGD.cu:90-102);
-// followed by each scene's decision (one wavefront per scene): a
-// fixed-order reduction of its partial sums, the Lipschitz test
-// (APGD.cu:236-264) and either a backtracking step or the acceptance logic
-// (APGD.cu:266-332).  (Folding the decision into the next P1, every
-// workgroup deciding redundantly, measured slower on MI355X: 25.8 us for
-// that P1 against 15.3 + 7.3 us for P1 and the decision kernel.)
+//   P3 (contacts): (N gamma_new)_c = F_c (tmp_A - tmp_B) and gamma_new .
+//       N gamma_new (FP64) or Delta . N gamma_new (mixed);
+// followed by each scene's decision: a fixed-order reduction of its partial
+// sums, the Lipschitz test (APGD.cu:236-264) and either a backtracking step
+// or the acceptance logic (APGD.cu:266-332).  Several scenes decide in a
+// kernel of their own (one workgroup per scene); a single scene's decision
+// is taken by every workgroup of the next pass's P1, from the same totals in
+// the same order, and the decide kernel runs once per chunk of passes.
 //
 // Work is cut into tiles of one scene (kTC rows or kTB bodies); a scene's
 // partial sums are kept per tile and reduced in tile order, so a scene gives
@@ -99,7 +100,7 @@ namespace {
 
 constexpr int kTC = 512;   // contact kernels: 8 wavefronts, one tile of rows
 constexpr int kTB = 256;   // body kernel: one tile of bodies
-constexpr int kTD = 64;    // decide kernels: one wavefront per scene
+constexpr int kTD = 64;    // setup and final decisions: one wavefront per scene
 constexpr int kBacktrackCap = 1100;  // L doubles past 1e308 long before this
 constexpr int kPassesPerChunk = 16;  // even: a chunk ends on parity 0
 static_assert(kPassesPerChunk % 2 == 0, "pass parity must return to 0 at a chunk end");
