@@ -170,6 +170,7 @@ struct Args {
   TM* delta;            // mixed solver: gamma_new - y
   double* part1[2];     // kSlots per tile, by pass parity
   double* part3[2];
+  double* q3[2];        // P3's one partial per tile (slot 6), packed, by pass parity
   double* p1sum[2];     // single slot: P1's slots totalled by P2's last workgroup
 };
 
@@ -291,7 +292,7 @@ __device__ void reduceAll(const double* part1, const double* part3, int ng, doub
 // and P3's slot 6 this way, wherever it takes them from.  In a wider
 // workgroup the threads past kTP add nothing, so the totals are the same.
 constexpr int kTP = 256;
-template <int NS>
+template <int NS, int kStride = kSlots>
 __device__ void reduceSlots(const double* part, int s0, int ng, double (&out)[NS]) {
   constexpr int K = 8, W = kTP / 64;
   __shared__ double sh[W][NS];
@@ -307,7 +308,7 @@ __device__ void reduceSlots(const double* part, int s0, int ng, double (&out)[NS
       const int w = base + (int)threadIdx.x + kTP * k;
       const bool on = mine && w < ng;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) x[k][s] = on ? gl(part)[(long)w * kSlots + s0 + s] : 0.0;
+      for (int s = 0; s < NS; ++s) x[k][s] = on ? gl(part)[(long)w * kStride + s0 + s] : 0.0;
     }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -332,27 +333,28 @@ __device__ void reduceSlots(const double* part, int s0, int ng, double (&out)[NS
 }
 
 // The totals the pass decision needs: P1's slots 0..5 and 7 (slot 6 is
-// unused) and P3's slot 6; the other slots are 0.  reducePass reduces P1's
-// partials; passTotals takes them already totalled (a single slot's P2
-// totals them), which keeps the registers of the eight-slot reduction out of
-// the kernels that decide a single slot.
-__device__ void reducePass(const double* part1, const double* part3, int ng, double* out) {
+// unused) and P3's slot 6 (its one partial per tile, packed in q3); the
+// other slots are 0.  reducePass reduces P1's partials; passTotals takes
+// them already totalled (a single slot's P2 totals them), which keeps the
+// registers of the eight-slot reduction out of the kernels that decide a
+// single slot.
+__device__ void reducePass(const double* part1, const double* q3, int ng, double* out) {
 #pragma unroll
   for (int s = 0; s < 2 * kSlots; ++s) out[s] = 0.0;
   double t1[kSlots], t3[1];
   reduceSlots<kSlots>(part1, 0, ng, t1);
-  reduceSlots<1>(part3, 6, ng, t3);
+  reduceSlots<1, 1>(q3, 0, ng, t3);
 #pragma unroll
   for (int s = 0; s < kSlots; ++s) out[s] = t1[s];
   out[kSlots + 6] = t3[0];
 }
-__device__ void passTotals(const double* p1, const double* part3, int ng, double* out) {
+__device__ void passTotals(const double* p1, const double* q3, int ng, double* out) {
 #pragma unroll
   for (int s = 0; s < 2 * kSlots; ++s) out[s] = 0.0;
   double t3[1];
 #pragma unroll
   for (int s = 0; s < kSlots; ++s) out[s] = gl(p1)[s];
-  reduceSlots<1>(part3, 6, ng, t3);
+  reduceSlots<1, 1>(q3, 0, ng, t3);
   out[kSlots + 6] = t3[0];
 }
 
@@ -830,7 +832,7 @@ __device__ PassView decidePrevious(const Args<TM>& a, const Tables& T, int par) 
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   ApgdState S = STATE(T, 0, par);
   double tot[2 * kSlots];
-  passTotals(a.p1sum[par ^ 1], a.part3[par ^ 1], a.nTiles, tot);
+  passTotals(a.p1sum[par ^ 1], a.q3[par ^ 1], a.nTiles, tot);
   if (!S.done) decideStep<kMixed>(S, tot, a.dims);
   const PassView v = viewOf(S);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1049,7 +1051,12 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
           acc[6] += g[u][0] * (double)n0 + g[u][1] * (double)n1 + g[u][2] * (double)n2;
       }
     }
-    writePartials<NT>(acc, TILE_PART(a.part3[par], tile));
+    // only slot 6 is used: one number per tile, packed so that the
+    // decision's loads of all tiles' partials touch few cache lines
+    double a6[1] = {acc[6]};
+    __shared__ double t6[1];
+    blockReduce<NT, 1>(a6, t6);
+    if (threadIdx.x == 0) gl(a.q3[par])[tile.start / kTC] = t6[0];
   };
   if (T.single) {  // as in kP1
     const PassView S = loadView(T, 0, par ^ 1);
@@ -1080,7 +1087,7 @@ __global__ __launch_bounds__(kTP) void kDecide(const Args<TM>* __restrict__ A, T
     const Args<TM>& a = A[0];
     ApgdState S = STATE(T, 0, par ^ 1);
     double tot[2 * kSlots];
-    passTotals(a.p1sum[par], a.part3[par], a.nTiles, tot);
+    passTotals(a.p1sum[par], a.q3[par], a.nTiles, tot);
     if (!S.done) decideStep<kMixed>(S, tot, a.dims);
     if (threadIdx.x == 0) {
       STATE(T, 0, par) = S;
@@ -1094,7 +1101,7 @@ __global__ __launch_bounds__(kTP) void kDecide(const Args<TM>* __restrict__ A, T
   ApgdState S = STATE(T, blockIdx.x, par ^ 1);
   if (!S.done) {
     double tot[2 * kSlots];
-    reducePass(a.part1[par], a.part3[par], a.nTiles, tot);
+    reducePass(a.part1[par], a.q3[par], a.nTiles, tot);
     decideStep<kMixed>(S, tot, a.dims);
   }
   if (threadIdx.x == 0) {
@@ -1365,7 +1372,7 @@ void prepareScene(Scene& s, Args<TM>& a) {
   const long RsCap = std::max<long>(3L * s.ncCap + s.nbil, 1);
   const long itemsCap = (long)s.ncCap + s.nbil;
   const long tilesCap = std::max<long>(1, (itemsCap + kTC - 1) / kTC);
-  s.partials.reserve((size_t)4 * kSlots * tilesCap + 2 * kSlots);
+  s.partials.reserve((size_t)4 * kSlots * tilesCap + 2 * kSlots + 2 * tilesCap);
   s.work.reserve((size_t)5 * RsCap + 16);  // ring[4], gHat: FP64 in both solvers
   // TM block: rn[4], r, delta, then J (per contact) and tmp (per body) of 4
   // components 32-byte aligned, then (mixed) M^-1
@@ -1419,6 +1426,7 @@ void prepareScene(Scene& s, Args<TM>& a) {
     a.part1[k] = s.partials.p + (long)(2 * k) * kSlots * tilesCap;
     a.part3[k] = s.partials.p + (long)(2 * k + 1) * kSlots * tilesCap;
     a.p1sum[k] = s.partials.p + 4L * kSlots * tilesCap + k * kSlots;
+    a.q3[k] = s.partials.p + 4L * kSlots * tilesCap + 2 * kSlots + k * tilesCap;
   }
 }
 
