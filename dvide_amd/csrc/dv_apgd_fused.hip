@@ -957,22 +957,15 @@ __global__ __launch_bounds__(kTC / R, kP1MinWaves<R>) void kP1(const Args<TM>* _
     writePartials<NT>(acc, TILE_PART(a.part1[par], tile));
   };
   if (T.single) {
-    // one slot: its view, table entry and tile count do not depend on the
-    // tile, so they are requested together before the tile.  The grid has a
-    // workgroup for every tile the slot's capacity can hold, so each takes
-    // at most one tile and nothing stays live across a tile loop (which held
-    // P1 above 80 VGPRs).
+    // one slot: the grid has a workgroup for every tile the slot's capacity
+    // can hold, so each takes one tile and nothing stays live across a tile
+    // loop (which held P1 above 80 VGPRs).  Tiles past the row count store
+    // only their partial sums, which no decision reads, so no branch waits
+    // on the tile count; workgroup 0 stores the decided state.
     const Args<TM>& a = A[0];
-    const int nt = T.counts[0];
-    PassView S;
-    if (kFold) {
-      if (blockIdx.x != 0 && (int)blockIdx.x >= nt) return;  // workgroup 0 stores the state
-      S = decidePrevious(a, T, par);
-    } else {
-      S = loadView(T, 0, par ^ 1);
-    }
+    const PassView S = kFold ? decidePrevious(a, T, par) : loadView(T, 0, par ^ 1);
     if (viewDone(S)) return;
-    if ((int)blockIdx.x < nt) tileBody(Tile{0, (int)blockIdx.x * kTC}, S, a);
+    tileBody(Tile{0, (int)blockIdx.x * kTC}, S, a);
     return;
   }
   TILE_LOOP(T) {
@@ -995,7 +988,10 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
     DV_GLOBAL TM* Nn = gl(a.rn[0] + S.nxt * a.ringStride);
     // loads of all contact rows of the thread, the body ids first: loads
     // return in order, so the tmp gathers, which need the ids, wait only for
-    // them and go out while the rest are still in flight
+    // them and go out while the rest are still in flight.  Every load is
+    // unconditional and every lane computes a contact row (rows not owned
+    // with contact 0's data and body 0), which only the owners store: a
+    // load under a row test would be a branch that waits for it.
     double g[R][3];
     TM dl[R][3], fc[R][6];
     int ca[R], cb[R];
@@ -1003,9 +999,11 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
     for (int u = 0; u < R; ++u) {
       const long q = (long)tile.start + threadIdx.x + u * NT;
       const long qc = q < D.nc ? q : 0;  // contact 0 stands in for rows not owned
-      // body 0 for rows not owned: the contact arrays may hold no contact
-      ca[u] = q < D.nc ? gl(a.cA)[qc] : 0;
-      cb[u] = q < D.nc ? gl(a.cB)[qc] : 0;
+      // the arrays hold at least one entry, but maybe no contact: rows not
+      // owned take body 0
+      const int ia = gl(a.cA)[qc], ib = gl(a.cB)[qc];
+      ca[u] = q < D.nc ? ia : 0;
+      cb[u] = q < D.nc ? ib : 0;
     }
 #pragma unroll
     for (int u = 0; u < R; ++u) {
@@ -1031,24 +1029,23 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       const long q = (long)tile.start + threadIdx.x + u * NT;
-      if (q >= items) continue;
-      if (q >= D.nc) {
+      const bool own = q < D.nc;
+      TM F[9];
+      frameFrom(fc[u], D.geomMode, F);
+      const TM d0 = tA[u].x - tB[u].x, d1 = tA[u].y - tB[u].y, d2 = tA[u].z - tB[u].z;
+      const TM n0 = F[0] * d0 + F[1] * d1 + F[2] * d2;
+      const TM n1 = F[3] * d0 + F[4] * d1 + F[5] * d2;
+      const TM n2 = F[6] * d0 + F[7] * d1 + F[8] * d2;
+      const double c6 = kMixed ? (double)dl[u][0] * (double)n0 + (double)dl[u][1] * (double)n1 + (double)dl[u][2] * (double)n2
+                               : g[u][0] * (double)n0 + g[u][1] * (double)n1 + g[u][2] * (double)n2;
+      if (own) {
+        Nn[ROW(a, 0, q)] = n0; Nn[ROW(a, 1, q)] = n1; Nn[ROW(a, 2, q)] = n2;
+        acc[6] += c6;
+      } else if (q < items) {
         const long f = BROW(a, q - D.nc);
         const TM nv = bilRow(a, (int)(q - D.nc));
         Nn[f] = nv;
         acc[6] += (kMixed ? (double)gl(a.delta)[f] : Gn[f]) * (double)nv;
-      } else {
-        TM F[9];
-        frameFrom(fc[u], D.geomMode, F);
-        const TM d0 = tA[u].x - tB[u].x, d1 = tA[u].y - tB[u].y, d2 = tA[u].z - tB[u].z;
-        const TM n0 = F[0] * d0 + F[1] * d1 + F[2] * d2;
-        const TM n1 = F[3] * d0 + F[4] * d1 + F[5] * d2;
-        const TM n2 = F[6] * d0 + F[7] * d1 + F[8] * d2;
-        Nn[ROW(a, 0, q)] = n0; Nn[ROW(a, 1, q)] = n1; Nn[ROW(a, 2, q)] = n2;
-        if (kMixed)
-          acc[6] += (double)dl[u][0] * (double)n0 + (double)dl[u][1] * (double)n1 + (double)dl[u][2] * (double)n2;
-        else
-          acc[6] += g[u][0] * (double)n0 + g[u][1] * (double)n1 + g[u][2] * (double)n2;
       }
     }
     // only slot 6 is used: one number per tile, packed so that the
@@ -1058,12 +1055,17 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
     blockReduce<NT, 1>(a6, t6);
     if (threadIdx.x == 0) gl(a.q3[par])[tile.start / kTC] = t6[0];
   };
-  if (T.single) {  // as in kP1
+  if (T.single) {
+    // one slot: every workgroup takes its tile (the grid covers the slot's
+    // tile capacity; tiles past the row count store only their partial,
+    // which no decision reads) with no test of the view's done flag, so
+    // that no branch waits on the view or the tile count before the loads.
+    // After the solve has ended the pass's stores are simply not used: N of
+    // ring slot nxt is never gamma-hat's, and the final kernels rebuild what
+    // they read.
     const PassView S = loadView(T, 0, par ^ 1);
     const Args<TM>& a = A[0];
-    const int nt = T.counts[0];
-    if (viewDone(S)) return;
-    for (int ti = blockIdx.x; ti < nt; ti += gridDim.x) tileBody(Tile{0, ti * kTC}, S, a);
+    tileBody(Tile{0, (int)blockIdx.x * kTC}, S, a);
     return;
   }
   TILE_LOOP(T) {
