@@ -846,8 +846,8 @@ __device__ __forceinline__ void frameFrom(const float (&c)[6], int, float (&F)[9
 // the FP64 instance spills a few registers to scratch), so that a CU keeps three
 // of its 512-thread workgroups in flight rather than two: past two tiles per
 // CU (262,144 rows on 256 CUs), the CUs given a third tile then run it beside
-// the other two instead of after one of them (P1 11.3 us instead of 12.7 us
-// at 263k contacts, 10.4 us against 10.2 us at 256k).
+// the other two instead of after one of them (P1 11% faster at 263k contacts,
+// 2% slower at 256k, where no CU takes a third tile).
 template <int R>
 constexpr int kP1MinWaves = R == 1 ? 6 : 1;
 
