@@ -438,7 +438,7 @@ def main():
         b_n = 112 * nc + 28 * mixed["nb"]
         t_n = kernel_us["P2"] + kernel_us["P3"]
         line["spmv_phase"] = {
-            "kernels": "kBody (D^T gather, M^-1) + kP3 (D gather, residual)",
+            "kernels": "kBody (D^T gather, M^-1) + kP3 (D gather, Delta . N gamma_new)",
             "bytes": b_n,
             "us": round(t_n, 3),
             "achieved_gbs": round(b_n / (t_n * 1e-6) / 1e9, 1),
