@@ -164,14 +164,18 @@ bool stepAssemble(Scene& s) {
 
 // Second half (System.cu:569-590): the update after the solve, time and
 // statistics.  The solve's results must be ordered before this on the
-// scene's stream.
-void stepFinish(Scene& s, bool rows, bool exactOrder) {
+// scene's stream.  stepFinishQueue only queues the update, so that a batch
+// can queue those of all its scenes before waiting on any.
+void stepFinishQueue(Scene& s, bool rows, bool exactOrder) {
   if (rows && !std::isfinite(s.stats.residual) && s.stats.iterations > 0)
     throw DvError{DV_ERR_DIVERGED, "APGD produced a non-finite residual"};
   const hipStream_t st = s.stream;
   DV_HIP(hipEventRecord(s.ev[3], st));
   updateState(s, rows, exactOrder);
   DV_HIP(hipEventRecord(s.ev[4], st));
+}
+
+void stepFinishWait(Scene& s) {
   DV_HIP(hipEventSynchronize(s.ev[4]));
   DV_HIP(hipGetLastError());
   s.time += s.h;
@@ -186,6 +190,11 @@ void stepFinish(Scene& s, bool rows, bool exactOrder) {
   s.stats.time = s.time;
   if (s.verbose)
     printf("Time: %f (Exec. Time: %f), Collisions: %d\n", s.time, s.stats.ms_total, s.nc);
+}
+
+void stepFinish(Scene& s, bool rows, bool exactOrder) {
+  stepFinishQueue(s, rows, exactOrder);
+  stepFinishWait(s);
 }
 
 void step(Scene& s) {
@@ -776,10 +785,19 @@ void batchStep(dv_batch& B) {
   DV_HIP(hipEventRecord(B.solved, B.stream));
   const double msSolve = msSince(t1);
   const auto t2 = std::chrono::steady_clock::now();
-  for (size_t i = 0; i < B.scenes.size(); ++i) {
+  for (size_t i = 0; i < B.scenes.size(); ++i) {  // a diverged solve fails the step before any update
+    const Scene& s = B.scenes[i]->s;
+    if (rows[i] && !std::isfinite(s.stats.residual) && s.stats.iterations > 0)
+      throw DvError{DV_ERR_DIVERGED, "APGD produced a non-finite residual"};
+  }
+  for (size_t i = 0; i < B.scenes.size(); ++i) {  // all updates queued first
     Scene& s = B.scenes[i]->s;
     DV_HIP(hipStreamWaitEvent(s.stream, B.solved, 0));
-    stepFinish(s, rows[i] != 0, false);
+    stepFinishQueue(s, rows[i] != 0, false);
+  }
+  for (size_t i = 0; i < B.scenes.size(); ++i) {
+    Scene& s = B.scenes[i]->s;
+    stepFinishWait(s);
     B.stats.total_contacts += s.nc;
     B.stats.total_iterations += s.stats.iterations;
     B.stats.max_iterations = std::max(B.stats.max_iterations, s.stats.iterations);
