@@ -11,7 +11,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <new>
+#include <thread>
 
 struct dv_scene {
   dv::Scene s;
@@ -759,6 +761,36 @@ double solverBytes(const Scene& s, bool mixed) {
          std::max(s.nb, 1) * (5 * tm) + 4.0 * 8 * 8 * (1 + (rows / 3 + 1023) / 1024);
 }
 
+// f(i) for i in [0, n) on up to eight host threads (each scene has its own
+// stream and buffers, and the HIP runtime takes launches from several
+// threads), so that the per-scene launch and read-back latencies of a
+// batch overlap.  The first exception is rethrown after all threads end.
+template <class F>
+void parallelScenes(size_t n, F&& f) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const unsigned nt = (unsigned)std::min<size_t>(std::min(8u, hw), n / 16);
+  if (nt <= 1) {
+    for (size_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  int dev = 0;
+  DV_HIP(hipGetDevice(&dev));  // a new thread's current device is the runtime's default
+  std::vector<std::thread> th;
+  std::vector<std::exception_ptr> err(nt);
+  for (unsigned t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      try {
+        DV_HIP(hipSetDevice(dev));
+        for (size_t i = t; i < n; i += nt) f(i);
+      } catch (...) {
+        err[t] = std::current_exception();
+      }
+    });
+  for (std::thread& x : th) x.join();
+  for (std::exception_ptr& e : err)
+    if (e) std::rethrow_exception(e);
+}
+
 void batchStep(dv_batch& B) {
   if (B.scenes.empty()) return;
   const auto t0 = std::chrono::steady_clock::now();
@@ -769,10 +801,11 @@ void batchStep(dv_batch& B) {
   B.stats = dv_batch_stats{};
   std::vector<Scene*> solve;
   std::vector<char> rows(B.scenes.size());
-  for (dv_scene* h : B.scenes) stepAssembleBegin(h->s);  // all counts queued first
+  // all counts queued first
+  parallelScenes(B.scenes.size(), [&](size_t i) { stepAssembleBegin(B.scenes[i]->s); });
+  parallelScenes(B.scenes.size(), [&](size_t i) { rows[i] = stepAssembleEnd(B.scenes[i]->s); });
   for (size_t i = 0; i < B.scenes.size(); ++i) {
     Scene& s = B.scenes[i]->s;
-    rows[i] = stepAssembleEnd(s);
     if (rows[i]) {
       DV_HIP(hipStreamWaitEvent(B.stream, s.ev[2], 0));
       solve.push_back(&s);

@@ -64,6 +64,25 @@ def test_batch_equals_alone(gpu, solver):
     batch.close()
 
 
+def test_batch_of_many_equals_alone(gpu):
+    """A batch large enough that its scenes' collision detection and
+    assembly are queued from several host threads gives the same results."""
+    solver = system.SOLVER_MIXED
+    cases = [("pile", 100 + i, 200 + 37 * i) for i in range(40)]
+    alone = [system.from_scene(make(*c), solver, tol=1e-5, max_iterations=3000) for c in cases]
+    together = [system.from_scene(make(*c), solver, tol=1e-5, max_iterations=3000) for c in cases]
+    batch = system.Batch(solver)
+    for g in together:
+        batch.add(g)
+    for k in range(2):
+        for g in alone:
+            g.step()
+        batch.step()
+        for i, (a, b) in enumerate(zip(alone, together)):
+            same_state(a, b, (k, i))
+    batch.close()
+
+
 @pytest.mark.parametrize("max_slots,budget", [(8, 0.0), (2, 0.0), (4, 1.0)])
 def test_server_mix_equals_alone(gpu, max_slots, budget):
     """Slots fewer than requests and a budget that admits one scene at a time
