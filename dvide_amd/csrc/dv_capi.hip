@@ -823,14 +823,14 @@ void batchStep(dv_batch& B) {
     if (rows[i] && !std::isfinite(s.stats.residual) && s.stats.iterations > 0)
       throw DvError{DV_ERR_DIVERGED, "APGD produced a non-finite residual"};
   }
-  for (size_t i = 0; i < B.scenes.size(); ++i) {  // all updates queued first
+  parallelScenes(B.scenes.size(), [&](size_t i) {  // all updates queued first
     Scene& s = B.scenes[i]->s;
     DV_HIP(hipStreamWaitEvent(s.stream, B.solved, 0));
     stepFinishQueue(s, rows[i] != 0, false);
-  }
+  });
+  parallelScenes(B.scenes.size(), [&](size_t i) { stepFinishWait(B.scenes[i]->s); });
   for (size_t i = 0; i < B.scenes.size(); ++i) {
-    Scene& s = B.scenes[i]->s;
-    stepFinishWait(s);
+    const Scene& s = B.scenes[i]->s;
     B.stats.total_contacts += s.nc;
     B.stats.total_iterations += s.stats.iterations;
     B.stats.max_iterations = std::max(B.stats.max_iterations, s.stats.iterations);
