@@ -1526,8 +1526,7 @@ Tables tablesOf(Engine& E, TileSet& set) {
 // on 256 CUs made one CU carry twice the rows of the others, and P1 took 35%
 // longer at 263k contacts than at 262k; 256-row tiles balance better still
 // but cost more in partial sums.  Past 512 tiles a CU holds three P1
-// workgroups at once (kP1MinWaves).  Several slots share two workgroups per
-// CU over their tiles.
+// workgroups at once (kP1MinWaves).
 template <typename TM>
 void grids(Engine& E, const TileSet& set, int& gc, int& gb) {
   if (E.nSlots == 1) {
@@ -1537,8 +1536,12 @@ void grids(Engine& E, const TileSet& set, int& gc, int& gb) {
     gc = (int)tilesCap;
     gb = (int)std::max<long>(1, std::min<long>((a.dims.nb + kTB - 1) / kTB, 4L * E.numCU));
   } else {
-    gc = 2 * E.numCU;
-    gb = 4 * E.numCU;
+    // several slots: 32 workgroups per CU over the tile tables, each taking
+    // a few tiles, so that the table and view reads that start every tile
+    // overlap across many workgroups (512 scenes: the solve 330 ms per batch
+    // step instead of 386 ms with 2 and 4 per CU; 64 per CU is slower again)
+    gc = 32 * E.numCU;
+    gb = 32 * E.numCU;
   }
   (void)set;
 }
