@@ -907,11 +907,19 @@ void serverPoll(dv_server& S) {
   if (!done.empty()) {
     engineRetire(S.engine, done, S.stream);
     DV_HIP(hipEventRecord(S.solved, S.stream));
+    for (int k : done) {  // a diverged solve fails the poll before any update
+      const Scene& s = S.slot[k].scene->s;
+      if (!std::isfinite(s.stats.residual) && s.stats.iterations > 0)
+        throw DvError{DV_ERR_DIVERGED, "APGD produced a non-finite residual"};
+    }
+    for (int k : done) {  // all updates queued first
+      Scene& s = S.slot[k].scene->s;
+      DV_HIP(hipStreamWaitEvent(s.stream, S.solved, 0));
+      stepFinishQueue(s, true, false);
+    }
     for (int k : done) {
       dv_server::Request r = S.slot[k];
-      Scene& s = r.scene->s;
-      DV_HIP(hipStreamWaitEvent(s.stream, S.solved, 0));
-      stepFinish(s, true, false);
+      stepFinishWait(r.scene->s);
       S.used -= r.bytes;
       S.completed.push_back(r.ticket);
       S.slot[k] = dv_server::Request{-1, nullptr, 0};
