@@ -877,27 +877,43 @@ bool inFlight(const dv_server& S, const dv_scene* h) {
 void serverPoll(dv_server& S) {
   if (engineBusy(S.engine)) engineRunChunk(S.engine, S.stream);
   // admit in submission order while a slot is free and the budget allows
-  // (an idle server always admits one request)
-  bool admitted = false;
+  // (an idle server always admits one request); the solver memory of a
+  // request is estimated from its scene's capacities before its assembly,
+  // so the requests are chosen first and then assembled together, from the
+  // host threads
+  std::vector<std::pair<dv_server::Request, int>> take;  // request, slot
+  double used = S.used;
+  const bool busy = engineBusy(S.engine);
   while (!S.waiting.empty()) {
     int free = -1;
-    for (int k = 0; k < S.maxSlots && free < 0; ++k)
-      if (S.slot[k].ticket < 0) free = k;
+    for (int k = 0; k < S.maxSlots && free < 0; ++k) {
+      bool taken = S.slot[k].ticket >= 0;
+      for (const auto& t : take) taken = taken || t.second == k;
+      if (!taken) free = k;
+    }
     if (free < 0) break;
     dv_server::Request r = S.waiting.front();
     r.bytes = solverBytes(r.scene->s, S.solver == DV_SOLVER_APGD_MIXED);
-    if (S.used + r.bytes > S.budget && (engineBusy(S.engine) || admitted)) break;
+    if (used + r.bytes > S.budget && (busy || !take.empty())) break;
     S.waiting.pop_front();
+    used += r.bytes;
+    take.emplace_back(r, free);
+  }
+  std::vector<char> rows(take.size());
+  parallelScenes(take.size(), [&](size_t i) { stepAssembleBegin(take[i].first.scene->s); });
+  parallelScenes(take.size(), [&](size_t i) { rows[i] = stepAssembleEnd(take[i].first.scene->s); });
+  bool admitted = false;
+  for (size_t i = 0; i < take.size(); ++i) {
+    const dv_server::Request& r = take[i].first;
     Scene& s = r.scene->s;
-    const bool rows = stepAssemble(s);
-    if (!rows) {  // nothing to solve: the update completes the request
+    if (!rows[i]) {  // nothing to solve: the update completes the request
       stepFinish(s, false, false);
       S.completed.push_back(r.ticket);
       continue;
     }
     DV_HIP(hipStreamWaitEvent(S.stream, s.ev[2], 0));
-    engineAdmit(S.engine, free, s);
-    S.slot[free] = r;
+    engineAdmit(S.engine, take[i].second, s);
+    S.slot[take[i].second] = r;
     S.used += r.bytes;
     admitted = true;
   }
