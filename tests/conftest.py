@@ -24,7 +24,7 @@ def _gpu_available():
 
 @pytest.fixture(scope="session")
 def gpu():
-    """Skip-free guard: GPU tests must run on a GPU box, so fail loudly without one."""
+    """Skip-free guard: a GPU test fails loudly where no GPU is visible instead of skipping."""
     if not _gpu_available():
         pytest.fail("GPU test selected but no GPU is visible")
     from dvide_amd import system
