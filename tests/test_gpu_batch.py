@@ -64,6 +64,30 @@ def test_batch_equals_alone(gpu, solver):
     batch.close()
 
 
+def test_batch_with_a_scene_without_contacts(gpu):
+    """A member with nothing to solve (free fall) is only updated, and the
+    others are solved as alone."""
+    solver = system.SOLVER_MIXED
+
+    def members():
+        return [system.from_scene(make("pile", 31, 900), solver, tol=1e-5, max_iterations=3000),
+                system.from_scene(scenes.free_fall(7, n=12), solver, tol=1e-5, max_iterations=3000),
+                system.from_scene(make("pile", 32, 400), solver, tol=1e-5, max_iterations=3000)]
+
+    alone, together = members(), members()
+    batch = system.Batch(solver)
+    for g in together:
+        batch.add(g)
+    for k in range(2):
+        for g in alone:
+            g.step()
+        batch.step()
+        assert together[1].stats().num_contacts == 0
+        for i, (a, b) in enumerate(zip(alone, together)):
+            same_state(a, b, (k, i))
+    batch.close()
+
+
 def test_batch_of_many_equals_alone(gpu):
     """A batch large enough that its scenes' collision detection and
     assembly are queued from several host threads gives the same results."""
