@@ -126,6 +126,29 @@ def test_server_mix_equals_alone(gpu, max_slots, budget):
     srv.close()
 
 
+def test_server_request_without_contacts(gpu):
+    """A request with nothing to solve completes at admission, beside
+    requests that are solved."""
+    solver = system.SOLVER_MIXED
+
+    def members():
+        return [system.from_scene(make("pile", 41, 700), solver, tol=1e-5, max_iterations=3000),
+                system.from_scene(scenes.free_fall(9, n=10), solver, tol=1e-5, max_iterations=3000),
+                system.from_scene(make("pile", 42, 500), solver, tol=1e-5, max_iterations=3000)]
+
+    alone, served = members(), members()
+    for g in alone:
+        g.step()
+    srv = system.Server(solver, max_slots=2)
+    tickets = [srv.submit(g) for g in served]
+    done = srv.drain()
+    assert sorted(done) == sorted(tickets)
+    assert served[1].stats().num_contacts == 0
+    for i, (a, b) in enumerate(zip(alone, served)):
+        same_state(a, b, i)
+    srv.close()
+
+
 def test_server_staggered_arrivals(gpu):
     """Requests submitted while others iterate join at chunk boundaries."""
     solver = system.SOLVER_FUSED
