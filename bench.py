@@ -62,7 +62,7 @@ def settled_scene(settle_steps=SETTLE_STEPS):
     from dvide_amd import scenes, system
 
     sc = scenes.compaction(2002, lid=scenes.LID_NONE)
-    g = system.from_scene(sc, system.SOLVER_MIXED, tol=TOL, max_iterations=20000)
+    g = system.from_scene(sc, system.BUILD_MIXED, tol=TOL, max_iterations=20000)
     its = []
     for _ in range(settle_steps):
         g.step()
@@ -165,7 +165,7 @@ def lid_pressed(steps=5):
 
     sc = scenes.compaction(2002)
     out = {}
-    for name, solver in (("mixed", system.SOLVER_MIXED), ("fp64", system.SOLVER_FUSED)):
+    for name, solver in (("mixed", system.BUILD_MIXED), ("fp64", system.BUILD_FP64)):
         g = system.from_scene(sc, solver, tol=TOL, max_iterations=20000)
         its, res, ms = [], [], []
         for _ in range(steps):
@@ -203,8 +203,8 @@ def batched(n_scenes=512, steps=2, warmup=1, seed=4004):
     t0 = time.perf_counter()
     scs = _gen_many([(seed + i, int(targets[i]), 0.03, 0.06, float(mus[i])) for i in range(n_scenes)])
     gen = time.perf_counter() - t0
-    systems = [system.from_scene(sc, system.SOLVER_MIXED, tol=TOL) for sc in scs]
-    batch = system.Batch(system.SOLVER_MIXED)
+    systems = [system.from_scene(sc, system.BUILD_MIXED, tol=TOL) for sc in scs]
+    batch = system.Batch(system.BUILD_MIXED)
     for g in systems:
         batch.add(g)
     for _ in range(warmup):
@@ -247,8 +247,8 @@ def served(n_scenes=256, n_requests=512, load=0.7, seed=5005, max_slots=64):
     rng = np.random.default_rng(seed)
     targets = np.exp(rng.uniform(np.log(1e3), np.log(2e5), n_scenes)).astype(int)
     scs = _gen_many([(seed + i, int(targets[i]), 0.02, 0.03, 0.4) for i in range(n_scenes)])
-    systems = [system.from_scene(sc, system.SOLVER_MIXED, tol=TOL) for sc in scs]
-    srv = system.Server(system.SOLVER_MIXED, max_slots=max_slots)
+    systems = [system.from_scene(sc, system.BUILD_MIXED, tol=TOL) for sc in scs]
+    srv = system.Server(system.BUILD_MIXED, max_slots=max_slots)
     # closed loop: every scene once, as fast as the server completes them
     t0 = time.perf_counter()
     for g in systems:
@@ -356,19 +356,19 @@ def main():
 
     scene, settle_its = settled_scene(args.settle)
     cap = args.max_iterations
-    mixed = run_gpu(scene, system.SOLVER_MIXED, args.steps, args.warmup, profile_kernel=system.PROFILE_P1,
+    mixed = run_gpu(scene, system.BUILD_MIXED, args.steps, args.warmup, profile_kernel=system.PROFILE_P1,
                     max_iterations=cap)
     # burst-timed averages of the body pass P2 and the contact pass P3 (two
     # untimed steps each from the same state): the Schur-product phase
     kernel_us = {"P1": None, "P2": None, "P3": None}
     for pk, name in ((system.PROFILE_P2, "P2"), (system.PROFILE_P3, "P3")):
-        r = run_gpu(scene, system.SOLVER_MIXED, 2, 0, profile_kernel=pk, max_iterations=cap)
+        r = run_gpu(scene, system.BUILD_MIXED, 2, 0, profile_kernel=pk, max_iterations=cap)
         if r["kernel_count"]:
             kernel_us[name] = round(1e3 * r["kernel_ms"] / r["kernel_count"], 3)
     others = {}
     if not args.only_mixed:
-        others["fp64"] = run_gpu(scene, system.SOLVER_FUSED, args.steps, args.warmup, max_iterations=cap)
-        others["plain_fp64"] = run_gpu(scene, system.SOLVER_PLAIN, max(2, args.steps // 4), args.warmup and 1,
+        others["fp64"] = run_gpu(scene, system.BUILD_FP64, args.steps, args.warmup, max_iterations=cap)
+        others["plain_fp64"] = run_gpu(scene, system.BUILD_PLAIN, max(2, args.steps // 4), args.warmup and 1,
                                        max_iterations=cap)
 
     value = args.steps / mixed["elapsed"]

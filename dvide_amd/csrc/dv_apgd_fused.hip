@@ -448,7 +448,10 @@ __device__ __forceinline__ double residualContact(double g0, double g1, double g
   double z0 = 1.0 * g0 + (-gdiff) * w0, z1 = 1.0 * g1 + (-gdiff) * w1, z2 = 1.0 * g2 + (-gdiff) * w2;
   projectCone<double>(z0, z1, z2, mu);
   const double e0 = ig * g0 + (-ig) * z0, e1 = ig * g1 + (-ig) * z1, e2 = ig * g2 + (-ig) * z2;
-  return fmax(fabs(e0), fmax(fabs(e1), fabs(e2)));
+  // fmax drops a NaN operand; a NaN row reports +inf, which every max keeps,
+  // so that non-finite data end the solve as diverged
+  const double sum = e0 + e1 + e2;
+  return sum == sum ? fmax(fabs(e0), fmax(fabs(e1), fabs(e2))) : INFINITY;
 }
 
 __device__ __forceinline__ void zero(double (&acc)[kSlots]) {
@@ -761,6 +764,11 @@ __device__ void decideStep(ApgdState& S, const double* tot, const SolveDims& D) 
   if (S.pendingRes) {
     S.pendingRes = 0;
     const double res = s1[kSlots - 1];
+    if (!(res < INFINITY)) {  // non-finite data: the solve ends as diverged
+      S.best = res;
+      S.done = 1;
+      return;
+    }
     if (res < S.best) {  // gammaHat := that iterate, kept in its ring slot
       S.best = res;
       S.hatSrc = S.cur;
@@ -951,7 +959,7 @@ __global__ __launch_bounds__(kTC / R, kP1MinWaves<R>) void kP1(const Args<TM>* _
           } else {
             e = fabs(w0);
           }
-          acc[7] = fmax(acc[7], e);
+          acc[7] = fmax(acc[7], e == e ? e : INFINITY);
         } else {
           const double w1 = (double)nk[u][1] + (double)rv[u][1];
           const double w2 = (double)nk[u][2] + (double)rv[u][2];
@@ -1307,14 +1315,12 @@ void timedBurst(const Plan<TM>& P, hipStream_t st, int kern, int par, bool fold,
     DV_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int r = 0; r < kProfBurst; ++r) launchPass<TM>(P, st, kern, par, fold);
     DV_HIP(hipStreamEndCapture(st, &graph));
-    hipError_t e = hipGraphInstantiate(&cache.exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) throw HipError{e, "hipGraphInstantiate(timed burst)"};
+    cache.instantiate(graph, "hipGraphInstantiate(timed burst)");
     cache.setKey(&key, sizeof(key));
     DV_HIP(hipGraphUpload(cache.exec, st));  // outside the timed region
   }
   DV_HIP(hipEventRecord(ev[0], st));
-  DV_HIP(hipGraphLaunch(cache.exec, st));
+  cache.launch(st);
   DV_HIP(hipEventRecord(ev[1], st));
 }
 
@@ -1352,9 +1358,25 @@ void enqueuePasses(const Plan<TM>& P, hipStream_t st, int n, hipEvent_t* ev, int
 }  // namespace
 
 void GraphCache::reset() {
-  if (exec) (void)hipGraphExecDestroy(exec);
+  if (exec) {
+    if (launchedOn) (void)hipStreamSynchronize(launchedOn);
+    (void)hipGraphExecDestroy(exec);
+  }
   exec = nullptr;
+  launchedOn = nullptr;
   key.clear();
+}
+void GraphCache::instantiate(hipGraph_t graph, const char* what) {
+  reset();
+  hipGraphExec_t x = nullptr;
+  const hipError_t e = hipGraphInstantiate(&x, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) throw HipError{e, what};
+  exec = x;
+}
+void GraphCache::launch(hipStream_t st) {
+  DV_HIP(hipGraphLaunch(exec, st));
+  launchedOn = st;
 }
 bool GraphCache::matches(const void* k, size_t n) const {
   return exec && key.size() == n && std::memcmp(key.data(), k, n) == 0;
@@ -1604,12 +1626,10 @@ void runChunk(Engine& E, hipStream_t st, hipEvent_t* ev, int prof) {
       DV_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       enqueuePasses<TM>(P, st, kPassesPerChunk, nullptr, 0, nullptr);
       DV_HIP(hipStreamEndCapture(st, &graph));
-      hipError_t e = hipGraphInstantiate(&E.graph.exec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (e != hipSuccess) throw HipError{e, "hipGraphInstantiate(apgd passes)"};
+      E.graph.instantiate(graph, "hipGraphInstantiate(apgd passes)");
       E.graph.setKey(&P, sizeof(P));
     }
-    DV_HIP(hipGraphLaunch(E.graph.exec, st));
+    E.graph.launch(st);
   }
   E.launched += kPassesPerChunk;
 }

@@ -24,6 +24,19 @@ __global__ __launch_bounds__(kT) void kAxpby(const double* __restrict__ x, const
   if (i < n) z[i] = a * x[i] + b * y[i];
 }
 
+// updateAntiRelaxationVector (APGD.cu:104-114): ar[3c] = mu |(s_u, s_v)| of
+// s = D v, ar[3c+1] = ar[3c+2] = 0, indexed from 0 without the bilateral
+// offset as the reference indexes it
+__global__ __launch_bounds__(kT) void kAntiRelax(const double* __restrict__ sv, double* __restrict__ ar, int nc,
+                                                 double mu) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const double a = sv[3 * c + 1], b = sv[3 * c + 2];
+  ar[3 * c] = sqrt(a * a + b * b) * mu;
+  ar[3 * c + 1] = 0.0;
+  ar[3 * c + 2] = 0.0;
+}
+
 __global__ __launch_bounds__(kT) void kInitGamma(double* g, int nc) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nc) return;
@@ -60,7 +73,8 @@ __global__ __launch_bounds__(kT) void kDot256(const double* __restrict__ x, cons
 __global__ __launch_bounds__(kT) void kMaxAbs(const double* __restrict__ x, long n, double* __restrict__ part) {
   __shared__ double t[kT];
   long i = (long)blockIdx.x * kT + threadIdx.x;
-  t[threadIdx.x] = i < n ? fabs(x[i]) : 0.0;
+  const double a = i < n ? fabs(x[i]) : 0.0;
+  t[threadIdx.x] = a == a ? a : INFINITY;  // fmax drops NaN; +inf is kept
   __syncthreads();
   for (int st = kT / 2; st >= 1; st >>= 1) {
     if ((int)threadIdx.x < st) t[threadIdx.x] = fmax(t[threadIdx.x], t[threadIdx.x + st]);
@@ -120,12 +134,13 @@ struct Plain {
 
 void solvePlain(Scene& s) {
   const long R = s.R;
-  // work vectors: gammaHat, gammaNew, g, y, yNew, t, Ny, Ng
-  s.work.reserve((size_t)8 * std::max<long>(R, 1));
+  // work vectors: gammaHat, gammaNew, g, y, yNew, t, Ny, Ng, the
+  // anti-relaxation vector
+  s.work.reserve((size_t)9 * std::max<long>(R, 1));
   double* W = s.work.p;
   const long S = std::max<long>(R, 1);
   double *gHat = W, *gNew = W + S, *g = W + 2 * S, *y = W + 3 * S, *yNew = W + 4 * S, *t = W + 5 * S,
-         *Ny = W + 6 * S, *Ng = W + 7 * S;
+         *Ny = W + 6 * S, *Ng = W + 7 * S, *ar = W + 8 * S;
   const int nblk = blocksFor(std::max<long>(R, 1), kT);
   s.partials.reserve(nblk);
   double* hp = nullptr;
@@ -140,6 +155,7 @@ void solvePlain(Scene& s) {
   if (!s.warm && s.nc)
     hipLaunchKernelGGL(kInitGamma, dim3(blocksFor(s.nc, kT)), dim3(kT), 0, s.stream, gamma, s.nc);
   fillDevice(s, gHat, R, 1.0);
+  if (s.antiRelax) fillDevice(s, ar, R, 0.0);  // APGD.cu:189
   P.copy(gamma, y);
   double theta = 1.0, thetaNew = 1.0, beta = 0, obj1 = 0, obj2 = 0, best = 10e30;
   P.axpby(gamma, gHat, t, 1.0, -1.0);
@@ -189,6 +205,10 @@ void solvePlain(Scene& s) {
       P.axpby(gNew, t, t, 1.0 / gdiff, -1.0 / gdiff);
     }
     double res = P.nrmmax(t);
+    if (!(res < INFINITY)) {  // non-finite data: the solve ends as diverged
+      best = res;
+      break;
+    }
     if (res < best) {
       best = res;
       P.copy(gNew, gHat);
@@ -204,6 +224,24 @@ void solvePlain(Scene& s) {
     theta = thetaNew;
     P.copy(gNew, gamma);
     P.copy(yNew, y);
+    if (s.antiRelax) {
+      // APGD.cu:335-345: r -= ar; f_contact = D^T gamma; v = M^-1 (k +
+      // f_contact); ar from D v; r += ar.  Like the reference it leaves v and
+      // f_contact overwritten (the update after the solve recomputes both)
+      // and r holding the last anti-relaxation term.
+      P.axpby(ar, s.r.p, s.r.p, -1.0, 1.0);
+      gatherExact(s, gamma, s.fContact.p, nullptr, false);
+      const long n3 = 3L * s.nb;
+      if (n3) {
+        hipLaunchKernelGGL(kAxpby, dim3(blocksFor(n3, kT)), dim3(kT), 0, s.stream, (const double*)s.k.p,
+                           (const double*)s.fContact.p, s.tmp.p, 1.0, 1.0, n3);
+        minvTimes(s, s.tmp.p, s.v.p);
+      }
+      rowsD(s, s.v.p, nullptr, t);
+      if (s.nc) hipLaunchKernelGGL(kAntiRelax, dim3(blocksFor(s.nc, kT)), dim3(kT), 0, s.stream, (const double*)t, ar,
+                                   s.nc, s.mu);
+      P.axpby(ar, s.r.p, s.r.p, 1.0, 1.0);
+    }
   }
   if (s.verbose) printf("  Iterations: %d Residual: %g\n", it, best);
   P.schur(gHat, t);

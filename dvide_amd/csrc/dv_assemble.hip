@@ -309,6 +309,10 @@ void schurExact(Scene& s, const double* x, double* y) {
                        (const double*)nullptr, y);
 }
 
+void minvTimes(Scene& s, const double* x, double* y) {
+  if (s.nb) hipLaunchKernelGGL(kMinv, dim3(blocksFor(3L * s.nb, kT)), dim3(kT), 0, s.stream, s.nb, s.minv.p, x, y);
+}
+
 void rowsD(Scene& s, const double* x, const double* badd, double* y) {
   const int items = s.nc + s.nbil;
   if (items)

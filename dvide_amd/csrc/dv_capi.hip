@@ -127,13 +127,21 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
+// The build that solves the scene: anti-relaxation runs on the plain build,
+// which a DV_APGD_FP64 scene then uses (the mixed build rejects it).
+int solveBuild(const Scene& s) {
+  if (s.antiRelax && s.build == DV_APGD_MIXED)
+    throw DvError{DV_ERR_UNSUPPORTED, "anti-relaxation runs on the FP64 builds, not the mixed one"};
+  return s.antiRelax ? DV_APGD_PLAIN : s.build;
+}
+
 void runSolver(Scene& s) {
-  switch (s.solver) {
-    case DV_SOLVER_APGD_PLAIN: solvePlain(s); break;
-    case DV_SOLVER_APGD_MIXED: solveFused(s, true); break;
+  switch (solveBuild(s)) {
+    case DV_APGD_PLAIN: solvePlain(s); break;
+    case DV_APGD_MIXED: solveFused(s, true); break;
     default: solveFused(s, false); break;
   }
-  if (!std::isfinite(s.stats.residual) && s.stats.iterations > 0)
+  if (!std::isfinite(s.stats.residual))
     throw DvError{DV_ERR_DIVERGED, "APGD produced a non-finite residual"};
 }
 
@@ -169,7 +177,7 @@ bool stepAssemble(Scene& s) {
 // scene's stream.  stepFinishQueue only queues the update, so that a batch
 // can queue those of all its scenes before waiting on any.
 void stepFinishQueue(Scene& s, bool rows, bool exactOrder) {
-  if (rows && !std::isfinite(s.stats.residual) && s.stats.iterations > 0)
+  if (rows && !std::isfinite(s.stats.residual))
     throw DvError{DV_ERR_DIVERGED, "APGD produced a non-finite residual"};
   const hipStream_t st = s.stream;
   DV_HIP(hipEventRecord(s.ev[3], st));
@@ -202,7 +210,7 @@ void stepFinish(Scene& s, bool rows, bool exactOrder) {
 void step(Scene& s) {
   const bool rows = stepAssemble(s);
   if (rows) runSolver(s);
-  stepFinish(s, rows, s.solver == DV_SOLVER_APGD_PLAIN);
+  stepFinish(s, rows, solveBuild(s) == DV_APGD_PLAIN);
 }
 
 }  // namespace
@@ -216,9 +224,16 @@ namespace {
 thread_local std::string g_createError;  // dv_last_error(NULL) after a failed create
 }
 
-int dv_scene_create(dv_scene** out, int solver) {
+int dv_scene_create(dv_scene** out, int solverType) {
   if (!out) return DV_ERR_ARG;
   *out = nullptr;
+  // System.cu:77-101: 2..7 are PDIP, TPAS, JKIP, PGJ, PGS and PJKIP
+  if (solverType >= 2 && solverType <= 7) {
+    static const char* names[] = {"PDIP", "TPAS", "JKIP", "PGJ", "PGS", "PJKIP"};
+    g_createError = std::string("solver type ") + std::to_string(solverType) + " (" + names[solverType - 2] +
+                    ") is not on this path; only APGD (1) is";
+    return DV_ERR_UNSUPPORTED;
+  }
   // no CPU fallback: without a HIP device there is nothing to run on
   int devices = 0;
   const hipError_t e = hipGetDeviceCount(&devices);
@@ -230,8 +245,6 @@ int dv_scene_create(dv_scene** out, int solver) {
     dv_scene* h = new dv_scene();
     const char* ng = getenv("DVIDE_NO_GRAPH");
     h->s.noGraph = ng && ng[0] == '1';
-    h->s.solver = (solver == DV_SOLVER_APGD_MIXED || solver == DV_SOLVER_APGD_PLAIN) ? solver
-                                                                                      : DV_SOLVER_APGD_FUSED;
     *out = h;
     return DV_OK;
   } catch (...) {
@@ -338,11 +351,12 @@ int dv_set_param(dv_scene* h, int param, double v) {
         s.envelope = v;  // the hash cell edge 2 (r_max + envelope) follows it each step
         break;
       case DV_PARAM_WARM_START: s.warm = v != 0; break;
-      case DV_PARAM_SOLVER:
-        if ((int)v != DV_SOLVER_APGD_FUSED && (int)v != DV_SOLVER_APGD_MIXED && (int)v != DV_SOLVER_APGD_PLAIN)
-          return dv::fail(s, DV_ERR_ARG, "unknown solver");
-        s.solver = (int)v;
+      case DV_PARAM_APGD_BUILD:
+        if ((int)v != DV_APGD_FP64 && (int)v != DV_APGD_MIXED && (int)v != DV_APGD_PLAIN)
+          return dv::fail(s, DV_ERR_ARG, "unknown APGD build");
+        s.build = (int)v;
         break;
+      case DV_PARAM_ANTI_RELAX: s.antiRelax = v != 0; break;
       case DV_PARAM_GEOM_MODE:
         if (v != 0 && v != 1) return dv::fail(s, DV_ERR_ARG, "geometry mode is 0 or 1");
         s.geomMode = (int)v;
@@ -355,7 +369,7 @@ int dv_set_param(dv_scene* h, int param, double v) {
         s.profKernel = (int)v;
         break;
       case DV_PARAM_ROWS_PER_THREAD:
-        if (v != 1 && v != 2) return dv::fail(s, DV_ERR_ARG, "rows per thread is 1 or 2");
+        if (v != 0 && v != 1 && v != 2) return dv::fail(s, DV_ERR_ARG, "rows per thread is 0 (tuned), 1 or 2");
         s.rowsPerThread = (int)v;
         break;
       default: return dv::fail(s, DV_ERR_ARG, "unknown parameter");
@@ -376,7 +390,8 @@ int dv_get_param(const dv_scene* hc, int param, double* v) {
       case DV_PARAM_GX: case DV_PARAM_GY: case DV_PARAM_GZ: *v = s.g[param - DV_PARAM_GX]; break;
       case DV_PARAM_ENVELOPE: *v = s.envelope; break;
       case DV_PARAM_WARM_START: *v = s.warm ? 1 : 0; break;
-      case DV_PARAM_SOLVER: *v = s.solver; break;
+      case DV_PARAM_APGD_BUILD: *v = s.build; break;
+      case DV_PARAM_ANTI_RELAX: *v = s.antiRelax ? 1 : 0; break;
       case DV_PARAM_GEOM_MODE: *v = s.geomMode; break;
       case DV_PARAM_TIME: *v = s.time; break;
       case DV_PARAM_VERBOSE: *v = s.verbose; break;
@@ -687,7 +702,7 @@ int dv_solve(dv_scene* h) {
 // batches and serving
 
 struct dv_batch {
-  int solver = DV_SOLVER_APGD_MIXED;
+  int build = DV_APGD_MIXED;
   std::vector<dv_scene*> scenes;
   dv::Engine engine;
   hipStream_t stream = nullptr;
@@ -702,7 +717,7 @@ struct dv_server {
     dv_scene* scene;
     double bytes;
   };
-  int solver = DV_SOLVER_APGD_MIXED;
+  int build = DV_APGD_MIXED;
   int maxSlots = 0;
   double budget = 0, used = 0;
   dv::Engine engine;
@@ -712,6 +727,7 @@ struct dv_server {
   std::deque<Request> waiting;
   std::vector<Request> slot;    // ticket < 0: free
   std::deque<long> completed;   // not yet reported
+  std::deque<long> failed;      // diverged, not yet reported (dv_server_failed)
   long nextTicket = 1;
   std::string err;
 };
@@ -741,14 +757,16 @@ double msSince(const std::chrono::steady_clock::time_point& t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-void needSolver(int solver) {
-  if (solver != DV_SOLVER_APGD_FUSED && solver != DV_SOLVER_APGD_MIXED)
-    throw DvError{DV_ERR_UNSUPPORTED, "batches and servers run the graph-driven solvers (FUSED or MIXED)"};
+void needBuild(int build) {
+  if (build != DV_APGD_FP64 && build != DV_APGD_MIXED)
+    throw DvError{DV_ERR_UNSUPPORTED, "batches and servers run the graph-driven builds (FP64 or MIXED)"};
 }
 
 void needScene(dv_scene* h) {
   if (!h) throw DvError{DV_ERR_ARG, "null scene"};
   if (!h->s.initialized) throw DvError{DV_ERR_STATE, "scene not initialized"};
+  if (h->s.antiRelax)
+    throw DvError{DV_ERR_UNSUPPORTED, "anti-relaxation runs on the plain build, not in batches or servers"};
 }
 
 // Device memory the solver of a slot needs beyond the scene's own state:
@@ -813,14 +831,14 @@ void batchStep(dv_batch& B) {
   }
   const double msAsm = msSince(t0);
   const auto t1 = std::chrono::steady_clock::now();
-  solveFusedMany(B.engine, solve, B.solver == DV_SOLVER_APGD_MIXED, B.stream, B.scenes[0]->s.numCU,
+  solveFusedMany(B.engine, solve, B.build == DV_APGD_MIXED, B.stream, B.scenes[0]->s.numCU,
                  B.scenes[0]->s.noGraph);
   DV_HIP(hipEventRecord(B.solved, B.stream));
   const double msSolve = msSince(t1);
   const auto t2 = std::chrono::steady_clock::now();
   for (size_t i = 0; i < B.scenes.size(); ++i) {  // a diverged solve fails the step before any update
     const Scene& s = B.scenes[i]->s;
-    if (rows[i] && !std::isfinite(s.stats.residual) && s.stats.iterations > 0)
+    if (rows[i] && !std::isfinite(s.stats.residual))
       throw DvError{DV_ERR_DIVERGED, "APGD produced a non-finite residual"};
   }
   parallelScenes(B.scenes.size(), [&](size_t i) {  // all updates queued first
@@ -842,10 +860,10 @@ void batchStep(dv_batch& B) {
   B.stats.ms_total = msSince(t0);
 }
 
-void serverOpen(dv_server& S, int solver, int maxSlots, double budget) {
-  needSolver(solver);
+void serverOpen(dv_server& S, int build, int maxSlots, double budget) {
+  needBuild(build);
   if (maxSlots < 2) throw DvError{DV_ERR_ARG, "a server needs at least two slots"};
-  S.solver = solver;
+  S.build = build;
   S.maxSlots = maxSlots;
   int dev = 0;
   DV_HIP(hipGetDevice(&dev));
@@ -859,7 +877,7 @@ void serverOpen(dv_server& S, int solver, int maxSlots, double budget) {
   DV_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
   DV_HIP(hipEventCreateWithFlags(&S.solved, hipEventDisableTiming));
   S.slot.assign(maxSlots, dv_server::Request{-1, nullptr, 0});
-  engineOpen(S.engine, maxSlots, solver == DV_SOLVER_APGD_MIXED, S.numCU, false, 1);
+  engineOpen(S.engine, maxSlots, build == DV_APGD_MIXED, S.numCU, false, 1);
 }
 
 bool inFlight(const dv_server& S, const dv_scene* h) {
@@ -893,15 +911,21 @@ void serverPoll(dv_server& S) {
     }
     if (free < 0) break;
     dv_server::Request r = S.waiting.front();
-    r.bytes = solverBytes(r.scene->s, S.solver == DV_SOLVER_APGD_MIXED);
+    r.bytes = solverBytes(r.scene->s, S.build == DV_APGD_MIXED);
     if (used + r.bytes > S.budget && (busy || !take.empty())) break;
     S.waiting.pop_front();
     used += r.bytes;
     take.emplace_back(r, free);
   }
   std::vector<char> rows(take.size());
-  parallelScenes(take.size(), [&](size_t i) { stepAssembleBegin(take[i].first.scene->s); });
-  parallelScenes(take.size(), [&](size_t i) { rows[i] = stepAssembleEnd(take[i].first.scene->s); });
+  try {
+    parallelScenes(take.size(), [&](size_t i) { stepAssembleBegin(take[i].first.scene->s); });
+    parallelScenes(take.size(), [&](size_t i) { rows[i] = stepAssembleEnd(take[i].first.scene->s); });
+  } catch (...) {
+    // nothing was admitted: the taken requests wait again, in their order
+    for (size_t i = take.size(); i-- > 0;) S.waiting.push_front(take[i].first);
+    throw;
+  }
   bool admitted = false;
   for (size_t i = 0; i < take.size(); ++i) {
     const dv_server::Request& r = take[i].first;
@@ -920,27 +944,36 @@ void serverPoll(dv_server& S) {
   if (admitted) engineStartAdmitted(S.engine, S.stream);
   // retire: gamma := gammaHat, stats, then each scene's update
   std::vector<int> done = enginePoll(S.engine, S.stream);
-  if (!done.empty()) {
-    engineRetire(S.engine, done, S.stream);
-    DV_HIP(hipEventRecord(S.solved, S.stream));
-    for (int k : done) {  // a diverged solve fails the poll before any update
-      const Scene& s = S.slot[k].scene->s;
-      if (!std::isfinite(s.stats.residual) && s.stats.iterations > 0)
-        throw DvError{DV_ERR_DIVERGED, "APGD produced a non-finite residual"};
-    }
-    for (int k : done) {  // all updates queued first
-      Scene& s = S.slot[k].scene->s;
-      DV_HIP(hipStreamWaitEvent(s.stream, S.solved, 0));
-      stepFinishQueue(s, true, false);
-    }
-    for (int k : done) {
-      dv_server::Request r = S.slot[k];
-      stepFinishWait(r.scene->s);
-      S.used -= r.bytes;
-      S.completed.push_back(r.ticket);
-      S.slot[k] = dv_server::Request{-1, nullptr, 0};
+  if (done.empty()) return;
+  engineRetire(S.engine, done, S.stream);
+  DV_HIP(hipEventRecord(S.solved, S.stream));
+  // every retired slot is freed; a solve with a non-finite residual fails
+  // its request (no update: the scene keeps its state before the step)
+  std::vector<dv_server::Request> ok;
+  std::string failed;
+  for (int k : done) {
+    const dv_server::Request r = S.slot[k];
+    S.slot[k] = dv_server::Request{-1, nullptr, 0};
+    S.used -= r.bytes;
+    const Scene& s = r.scene->s;
+    if (!std::isfinite(s.stats.residual)) {
+      S.failed.push_back(r.ticket);
+      failed += (failed.empty() ? "" : ", ") + std::to_string(r.ticket);
+    } else {
+      ok.push_back(r);
     }
   }
+  for (const auto& r : ok) {  // all updates queued first
+    Scene& s = r.scene->s;
+    DV_HIP(hipStreamWaitEvent(s.stream, S.solved, 0));
+    stepFinishQueue(s, true, false);
+  }
+  for (const auto& r : ok) {
+    stepFinishWait(r.scene->s);
+    S.completed.push_back(r.ticket);
+  }
+  if (!failed.empty())
+    throw DvError{DV_ERR_DIVERGED, "APGD produced a non-finite residual for request(s) " + failed};
 }
 
 }  // namespace
@@ -948,13 +981,13 @@ void serverPoll(dv_server& S) {
 
 extern "C" {
 
-int dv_batch_create(dv_batch** out, int solver) {
+int dv_batch_create(dv_batch** out, int build) {
   if (!out) return DV_ERR_ARG;
   *out = nullptr;
-  if (solver != DV_SOLVER_APGD_FUSED && solver != DV_SOLVER_APGD_MIXED) return DV_ERR_UNSUPPORTED;
+  if (build != DV_APGD_FP64 && build != DV_APGD_MIXED) return DV_ERR_UNSUPPORTED;
   try {
     dv_batch* b = new dv_batch();
-    b->solver = solver;
+    b->build = build;
     *out = b;
     return DV_OK;
   } catch (...) {
@@ -997,13 +1030,13 @@ int dv_batch_get_stats(const dv_batch* b, dv_batch_stats* out) {
   return DV_OK;
 }
 
-int dv_server_create(dv_server** out, int solver, int max_slots, double hbm_budget) {
+int dv_server_create(dv_server** out, int build, int max_slots, double hbm_budget) {
   if (!out) return DV_ERR_ARG;
   *out = nullptr;
   dv_server* srv = new (std::nothrow) dv_server();
   if (!srv) return DV_ERR_NOMEM;
   const int rc = dv::guardedH(srv, [&](dv_server& S) {
-    dv::serverOpen(S, solver, max_slots, hbm_budget);
+    dv::serverOpen(S, build, max_slots, hbm_budget);
     return DV_OK;
   });
   if (rc != DV_OK) {
@@ -1039,16 +1072,31 @@ int dv_server_submit(dv_server* srv, dv_scene* scene, long* ticket) {
 }
 
 int dv_server_poll(dv_server* srv, long* done, int max_done, int* n_done) {
-  return dv::guardedH(srv, [&](dv_server& S) {
+  if (n_done) *n_done = 0;
+  const int rc = dv::guardedH(srv, [&](dv_server& S) {
     dv::serverPoll(S);
-    int n = 0;
-    while (!S.completed.empty() && n < max_done && done) {
-      done[n++] = S.completed.front();
-      S.completed.pop_front();
-    }
-    if (n_done) *n_done = n;
     return DV_OK;
   });
+  if (!srv) return rc;
+  // completed requests are reported whether or not the poll failed
+  int n = 0;
+  while (!srv->completed.empty() && n < max_done && done) {
+    done[n++] = srv->completed.front();
+    srv->completed.pop_front();
+  }
+  if (n_done) *n_done = n;
+  return rc;
+}
+
+int dv_server_failed(dv_server* srv, long* tickets, int max_n, int* n_out) {
+  if (!srv || !n_out) return DV_ERR_ARG;
+  int n = 0;
+  while (!srv->failed.empty() && n < max_n && tickets) {
+    tickets[n++] = srv->failed.front();
+    srv->failed.pop_front();
+  }
+  *n_out = n;
+  return DV_OK;
 }
 
 int dv_server_pending(const dv_server* srv, int* n) {

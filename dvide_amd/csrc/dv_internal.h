@@ -75,12 +75,19 @@ struct SolveDims;
 struct Scene;
 
 // An instantiated hipGraph and the launch parameters it was captured with.
+// The stream it was last launched on is kept, so that an exec is destroyed
+// only after its last launch has finished.
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
+  hipStream_t launchedOn = nullptr;
   std::vector<char> key;
   void reset();
   bool matches(const void* k, size_t n) const;
   void setKey(const void* k, size_t n);
+  // instantiates `graph` (destroyed here) as the cached exec; the cache is
+  // left empty if instantiation fails
+  void instantiate(hipGraph_t graph, const char* what);
+  void launch(hipStream_t st);
 };
 
 // A tile of solver work: rows (or bodies) [start, start + tile size) of one
@@ -136,13 +143,14 @@ struct Scene {
   double g[3] = {0, -9.81, 0};
   int bins[3] = {20, 20, 20};
   bool warm = false;
-  int solver = DV_SOLVER_APGD_FUSED;
+  bool antiRelax = false;   // APGD::useAntiRelaxation (APGD.cu:15)
+  int build = DV_APGD_FP64; // DV_PARAM_APGD_BUILD
   int geomMode = 1;
   int verbose = 0;
   int grid = 0;
   bool noGraph = false;  // launch the solver passes directly (profiling)
   int profKernel = 0;    // DV_PARAM_PROFILE
-  int rowsPerThread = 1; // DV_PARAM_ROWS_PER_THREAD
+  int rowsPerThread = 0; // DV_PARAM_ROWS_PER_THREAD (0: the tuned choice)
   std::vector<hipEvent_t> profEv;
   double time = 0;
   int timeIndex = 0;
@@ -223,6 +231,7 @@ void gatherExact(Scene& s, const double* x, double* fOut, double* tmpOut, bool s
 void ensureRowBuffers(Scene& s);
 void fillDevice(Scene& s, double* x, long n, double v);
 void rowsD(Scene& s, const double* x, const double* badd, double* y);
+void minvTimes(Scene& s, const double* x, double* y);  // y = M^-1 x per DOF
 
 // solvers (dv_apgd_plain.hip, dv_apgd_fused.hip)
 void solvePlain(Scene& s);

@@ -20,10 +20,12 @@ LIB_PATH = os.path.join(_HERE, "libdvide_hip.so")
 
 # include/dvide.h
 DV_OK = 0
-SOLVER_FUSED, SOLVER_MIXED, SOLVER_PLAIN = 1, 2, 3
+DV_ERR_UNSUPPORTED = -5
+SOLVER_APGD = 1  # System(int solverType): 2..7 (PDIP .. PJKIP) are rejected
+BUILD_FP64, BUILD_MIXED, BUILD_PLAIN = 0, 1, 2  # DV_APGD_*
 (PARAM_H, PARAM_MU, PARAM_TOL, PARAM_MAX_ITER, PARAM_GX, PARAM_GY, PARAM_GZ, PARAM_ENVELOPE,
- PARAM_WARM_START, PARAM_SOLVER, PARAM_GEOM_MODE, PARAM_TIME, PARAM_VERBOSE, PARAM_GRID,
- PARAM_PROFILE, PARAM_ROWS_PER_THREAD) = range(16)
+ PARAM_WARM_START, PARAM_APGD_BUILD, PARAM_GEOM_MODE, PARAM_TIME, PARAM_VERBOSE, PARAM_GRID,
+ PARAM_PROFILE, PARAM_ROWS_PER_THREAD, PARAM_ANTI_RELAX) = range(17)
 PROFILE_OFF, PROFILE_P1, PROFILE_P2, PROFILE_P3, PROFILE_DECIDE = range(5)
 VEC_P, VEC_V, VEC_GAMMA, VEC_F_CONTACT, VEC_R, VEC_K, VEC_B = range(7)
 
@@ -36,7 +38,7 @@ EXPORTED = (
     "dv_batch_create", "dv_batch_destroy", "dv_batch_last_error", "dv_batch_add", "dv_batch_step",
     "dv_batch_get_stats",
     "dv_server_create", "dv_server_destroy", "dv_server_last_error", "dv_server_submit", "dv_server_poll",
-    "dv_server_pending",
+    "dv_server_pending", "dv_server_failed",
 )
 
 
@@ -133,6 +135,7 @@ def load_library(path: str = LIB_PATH):
         "dv_server_poll": ([vp, ctypes.POINTER(ctypes.c_long), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
                            ctypes.c_int),
         "dv_server_pending": ([vp, _ip], ctypes.c_int),
+        "dv_server_failed": ([vp, ctypes.POINTER(ctypes.c_long), ctypes.c_int, _ip], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -210,19 +213,25 @@ class Solver:
     def setWarmStarting(self, flag):  # APGD::setWarmStarting (APGD.cuh:62)
         self._sys._set(PARAM_WARM_START, 1.0 if flag else 0.0)
 
+    def setAntiRelaxation(self, flag):  # APGD::setAntiRelaxation (APGD.cuh:63)
+        self._sys._set(PARAM_ANTI_RELAX, 1.0 if flag else 0.0)
+
 
 class System:
     """Drop-in mirror of the reference's System (System.cuh:279-326)."""
 
-    def __init__(self, solverType: int = SOLVER_FUSED):
+    def __init__(self, solverType: int = SOLVER_APGD, build: int = BUILD_FP64):
         L = load_library()
         self._L = L
         h = ctypes.c_void_p()
+        self._h = None
         self._check(L.dv_scene_create(ctypes.byref(h), int(solverType)), None)
         self._h = h
         self.bodies: List[Body] = []
         self.solver = Solver(self)
         self._initialized = False
+        if build != BUILD_FP64:
+            self.setBuild(build)
 
     # -- plumbing
     def _check(self, rc, h="self"):
@@ -274,14 +283,15 @@ class System:
     def setGeometryMode(self, mode):
         self._set(PARAM_GEOM_MODE, mode)
 
-    def setSolver(self, solver):
-        self._set(PARAM_SOLVER, solver)
+    def setBuild(self, build):
+        """APGD build: BUILD_FP64, BUILD_MIXED or BUILD_PLAIN."""
+        self._set(PARAM_APGD_BUILD, build)
 
     def setGrid(self, g):
         self._set(PARAM_GRID, g)
 
     def setRowsPerThread(self, r):
-        """Contacts per thread of the solver's contact kernels (1 or 2)."""
+        """Contacts per thread of the solver's contact kernels (0 tuned, 1 or 2)."""
         self._set(PARAM_ROWS_PER_THREAD, r)
 
     def setProfileKernel(self, k):
@@ -484,7 +494,7 @@ def project_cone(g, mu):
     L = load_library()
     g = np.ascontiguousarray(g, np.float64).copy()
     h = ctypes.c_void_p()
-    rc = L.dv_scene_create(ctypes.byref(h), SOLVER_FUSED)
+    rc = L.dv_scene_create(ctypes.byref(h), SOLVER_APGD)
     if rc != DV_OK:
         raise DvideError(rc, "scene create")
     try:
@@ -496,15 +506,17 @@ def project_cone(g, mu):
     return g
 
 
-def from_scene(scene, solver: int = SOLVER_FUSED, *, tol=None, max_iterations=None, h=None,
-               mu=None, warm_start=None, geom_mode: int = 1, grid: Optional[int] = None) -> System:
-    """Build and initialise a System from a generated scene (dvide_amd.scenes)."""
+def from_scene(scene, build: int = BUILD_FP64, *, tol=None, max_iterations=None, h=None,
+               mu=None, warm_start=None, anti_relax=False, geom_mode: int = 1,
+               grid: Optional[int] = None) -> System:
+    """Build and initialise an APGD System in the given build from a generated
+    scene (dvide_amd.scenes)."""
     prm = dict(scene.params)
     for k, v in (("tol", tol), ("max_iterations", max_iterations), ("h", h), ("mu", mu),
                  ("warm_start", warm_start)):
         if v is not None:
             prm[k] = v
-    s = System(solver)
+    s = System(SOLVER_APGD, build)
     s.setTimeStep(prm["h"])
     s.setFrictionCoefficient(prm["mu"])
     s.setGravity(prm["gravity"])
@@ -513,6 +525,8 @@ def from_scene(scene, solver: int = SOLVER_FUSED, *, tol=None, max_iterations=No
     s.solver.tolerance = prm["tol"]
     s.solver.maxIterations = prm["max_iterations"]
     s.solver.setWarmStarting(prm["warm_start"])
+    if anti_relax:
+        s.solver.setAntiRelaxation(True)
     if grid:
         s.setGrid(grid)
     s.add_bodies(scene.pos, scene.vel, scene.mass, scene.geom, scene.fixed, scene.family)
@@ -527,11 +541,11 @@ class Batch:
     graph launches (dv_batch_*).  Each member System keeps its own state and
     statistics; its result equals stepping it alone with the same solver."""
 
-    def __init__(self, solver: int = SOLVER_MIXED):
+    def __init__(self, build: int = BUILD_MIXED):
         L = load_library()
         self._L = L
         h = ctypes.c_void_p()
-        rc = L.dv_batch_create(ctypes.byref(h), int(solver))
+        rc = L.dv_batch_create(ctypes.byref(h), int(build))
         if rc != DV_OK:
             raise DvideError(rc, "dv_batch_create")
         self._h = h
@@ -572,11 +586,11 @@ class Server:
     waiting requests by free slot and HBM budget, runs one chunk of solver
     passes and returns the tickets of completed requests."""
 
-    def __init__(self, solver: int = SOLVER_MIXED, max_slots: int = 64, hbm_budget: float = 0.0):
+    def __init__(self, build: int = BUILD_MIXED, max_slots: int = 64, hbm_budget: float = 0.0):
         L = load_library()
         self._L = L
         h = ctypes.c_void_p()
-        rc = L.dv_server_create(ctypes.byref(h), int(solver), int(max_slots), float(hbm_budget))
+        rc = L.dv_server_create(ctypes.byref(h), int(build), int(max_slots), float(hbm_budget))
         if rc != DV_OK:
             msg = L.dv_server_last_error(None)
             raise DvideError(rc, msg.decode() if msg else "")
@@ -594,8 +608,24 @@ class Server:
         return t.value
 
     def poll(self) -> List[int]:
+        """Tickets completed by this poll.  A diverged request raises
+        DvideError (DV_ERR_DIVERGED) after its slot is freed; the tickets the
+        poll completed are then on the error (.done) and the diverged ones
+        are returned by failed()."""
         n = ctypes.c_int()
-        self._check(self._L.dv_server_poll(self._h, self._buf, len(self._buf), ctypes.byref(n)))
+        rc = self._L.dv_server_poll(self._h, self._buf, len(self._buf), ctypes.byref(n))
+        done = [self._buf[i] for i in range(n.value)]
+        if rc != DV_OK:
+            msg = self._L.dv_server_last_error(self._h)
+            err = DvideError(rc, msg.decode() if msg else "")
+            err.done = done
+            raise err
+        return done
+
+    def failed(self) -> List[int]:
+        """Tickets of diverged requests not yet reported."""
+        n = ctypes.c_int()
+        self._check(self._L.dv_server_failed(self._h, self._buf, len(self._buf), ctypes.byref(n)))
         return [self._buf[i] for i in range(n.value)]
 
     def pending(self) -> int:

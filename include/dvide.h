@@ -41,11 +41,17 @@ typedef struct dv_scene dv_scene;
 #define DV_ERR_UNSUPPORTED (-5) /* option not supported by the selected solver */
 #define DV_ERR_DIVERGED (-6)    /* solver produced a non-finite value */
 
-/* solver implementations (the reference's factory, System.cu:77-101, knows
- * 1=APGD .. 7=PJKIP; only APGD is on this path, in three builds) */
-#define DV_SOLVER_APGD_FUSED 1 /* device-resident state machine in a hipGraph, FP64 */
-#define DV_SOLVER_APGD_MIXED 2 /* same, FP32 D/M^-1/products, FP64 iterates and tests */
-#define DV_SOLVER_APGD_PLAIN 3 /* host-driven FP64, one kernel per CUSP operation */
+/* System(int solverType) (System.cu:77-101): 1 = APGD; 2..7 name PDIP, TPAS,
+ * JKIP, PGJ, PGS and PJKIP, which are not on this path, so dv_scene_create
+ * rejects them with DV_ERR_UNSUPPORTED; any other value is the reference's
+ * default case, APGD. */
+#define DV_SOLVER_APGD 1
+
+/* APGD builds (DV_PARAM_APGD_BUILD); each runs APGD::solve (APGD.cu:164-361) */
+#define DV_APGD_FP64 0  /* device-resident state machine in hipGraph chunks, FP64 (default) */
+#define DV_APGD_MIXED 1 /* same, FP32 D/M^-1/products, FP64 iterates, tests and update */
+#define DV_APGD_PLAIN 2 /* host-driven FP64, one kernel per CUSP operation; the build
+                           that runs anti-relaxation */
 
 /* double parameters (dv_set_param / dv_get_param) */
 #define DV_PARAM_H 0          /* time step              System::setTimeStep, System.cu:135 */
@@ -57,7 +63,8 @@ typedef struct dv_scene dv_scene;
 #define DV_PARAM_GZ 6
 #define DV_PARAM_ENVELOPE 7   /* collision envelope     CollisionDetector::setEnvelope, CollisionDetector.cu:525 */
 #define DV_PARAM_WARM_START 8 /* 0/1                    APGD::setWarmStarting, APGD.cuh:62 */
-#define DV_PARAM_SOLVER 9     /* DV_SOLVER_*            System(int solverType), System.cu:62 */
+#define DV_PARAM_APGD_BUILD 9 /* DV_APGD_*: FP64, mixed or plain (no counterpart: the
+                                 reference has one APGD, in FP64) */
 #define DV_PARAM_GEOM_MODE 10 /* 1: lengths/normals rounded through binary32 as the
                                  reference's double3 helpers compile under nvcc
                                  (helper_math_double.h:576-598); 0: plain FP64 */
@@ -76,9 +83,13 @@ typedef struct dv_scene dv_scene;
                                  kernel once, alone, where it runs: after the chunk's
                                  last pass for one scene); the rest run in the captured
                                  chunk graph */
-#define DV_PARAM_ROWS_PER_THREAD 15 /* contact kernels of the graph-driven solvers: 1 or 2
-                                       contacts per thread (a tuning choice; results agree
-                                       to rounding, each choice is deterministic) */
+#define DV_PARAM_ROWS_PER_THREAD 15 /* contact kernels of the graph-driven solvers: 0 the
+                                       tuned choice (default), 1 or 2 contacts per thread
+                                       (results agree to rounding, each choice is
+                                       deterministic) */
+#define DV_PARAM_ANTI_RELAX 16 /* 0/1                  APGD::setAntiRelaxation, APGD.cuh:63
+                                  (APGD.cu:104-114, 335-345); runs on DV_APGD_PLAIN, and a
+                                  DV_APGD_FP64 scene with it on is solved by that build */
 
 /* state vectors (dv_get_vector / dv_set_vector) */
 #define DV_VEC_P 0         /* System::p_d        3*nb */
@@ -110,10 +121,12 @@ typedef struct dv_stats {
 
 /* --- scene lifetime ------------------------------------------------------ */
 
-/* System::System(int solverType), System.cu:62-133.  Fails with DV_ERR_HIP
- * when no HIP device is visible (there is no CPU fallback); dv_last_error(NULL)
- * then names the cause. */
-int dv_scene_create(dv_scene** out, int solver);
+/* System::System(int solverType), System.cu:62-133.  solverType 2..7 (PDIP,
+ * TPAS, JKIP, PGJ, PGS, PJKIP) fail with DV_ERR_UNSUPPORTED; 1 and the
+ * reference's default case give APGD in the DV_APGD_FP64 build.  Fails with
+ * DV_ERR_HIP when no HIP device is visible (there is no CPU fallback);
+ * dv_last_error(NULL) then names the cause. */
+int dv_scene_create(dv_scene** out, int solverType);
 int dv_scene_destroy(dv_scene* s);
 const char* dv_last_error(const dv_scene* s);
 
@@ -209,7 +222,7 @@ int dv_solve(dv_scene* s);
  * No counterpart in the reference, which steps one System at a time.  Both
  * step initialized scenes that stay resident on the device (bodies, contacts,
  * warm-start gamma) and remain usable alone.  A scene's result is
- * bit-identical whether it is stepped alone (dv_step with the same solver) or
+ * bit-identical whether it is stepped alone (dv_step with the same build) or
  * in a batch or a server mix: its partial sums are formed per tile of its
  * own rows and reduced in a fixed order. */
 
@@ -227,8 +240,8 @@ typedef struct dv_batch_stats {
   double ms_update;
 } dv_batch_stats;
 
-/* solver: DV_SOLVER_APGD_FUSED (FP64) or DV_SOLVER_APGD_MIXED */
-int dv_batch_create(dv_batch** out, int solver);
+/* build: DV_APGD_FP64 or DV_APGD_MIXED (the graph-driven builds) */
+int dv_batch_create(dv_batch** out, int build);
 int dv_batch_destroy(dv_batch* b);
 const char* dv_batch_last_error(const dv_batch* b);
 /* adds an initialized scene; the batch does not own it */
@@ -246,7 +259,7 @@ int dv_batch_get_stats(const dv_batch* b, dv_batch_stats* out);
  * detection and assembly run at admission, its solve joins the slots
  * already iterating at the next chunk of passes, and it completes when its
  * APGD has converged (or hit maxIterations) and its update has run. */
-int dv_server_create(dv_server** out, int solver, int max_slots, double hbm_budget);
+int dv_server_create(dv_server** out, int build, int max_slots, double hbm_budget);
 int dv_server_destroy(dv_server* srv);
 const char* dv_server_last_error(const dv_server* srv);
 /* queues one DoTimeStep of `scene` (one request per scene in flight);
@@ -254,9 +267,15 @@ const char* dv_server_last_error(const dv_server* srv);
 int dv_server_submit(dv_server* srv, dv_scene* scene, long* ticket);
 /* admits waiting requests, runs one chunk of solver passes, completes the
  * requests whose solves are done; their tickets go to done[0..*n_done) (at
- * most max_done; the rest are reported by later polls) */
+ * most max_done; the rest are reported by later polls).  A request whose
+ * solve diverged (non-finite residual) frees its slot without an update and
+ * makes the poll return DV_ERR_DIVERGED (done[] is still filled); its ticket
+ * is then reported by dv_server_failed.  If assembling admitted requests
+ * fails, they wait again in their order and the poll returns the error. */
 int dv_server_poll(dv_server* srv, long* done, int max_done, int* n_done);
-/* number of requests waiting or in flight */
+/* tickets of diverged requests not yet reported (at most max_n) */
+int dv_server_failed(dv_server* srv, long* tickets, int max_n, int* n);
+/* number of requests waiting, in flight or completed and not yet reported */
 int dv_server_pending(const dv_server* srv, int* n);
 
 #ifdef __cplusplus

@@ -10,13 +10,20 @@
 //     host copies returned by p(), v(), gamma(), f_contact(), with setters
 //     for p and v;
 //   * failures raise dvide::Error (the reference returns 0 and carries on);
-//   * the solver types are the C ABI's: DV_SOLVER_APGD_FUSED (= 1, APGD),
-//     DV_SOLVER_APGD_MIXED, DV_SOLVER_APGD_PLAIN; anti-relaxation is not
-//     available on the GPU path.
+//   * System(int solverType) knows the reference's numbering (System.cu:77-101):
+//     1 and the default case are APGD; 2..7 (PDIP .. PJKIP) are not on this
+//     path and throw dvide::Error(DV_ERR_UNSUPPORTED);
+//   * APGD has three builds, chosen by APGD::setBuild (DV_APGD_FP64, the
+//     default; DV_APGD_MIXED; DV_APGD_PLAIN); anti-relaxation runs on the
+//     FP64 builds (the plain one solves such a scene).
 #ifndef DVIDE_SYSTEM_HPP_
 #define DVIDE_SYSTEM_HPP_
 
+#include <cmath>
+#include <cstdio>
 #include <fstream>
+#include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -82,12 +89,15 @@ class Solver {
   virtual ~Solver() = default;
 };
 
-// APGD.cuh:62-63
+// APGD.cuh:62-63, plus the choice of build (no counterpart in the
+// reference, whose APGD is FP64 only)
 class APGD : public Solver {
  public:
   void setWarmStarting(bool b) { warm = b; }
   void setAntiRelaxation(bool b) { antiRelax = b; }
+  void setBuild(int b) { build = b; }
   bool warm = false, antiRelax = false;
+  int build = DV_APGD_FP64;
 };
 
 // CollisionDetector.cuh:89-101
@@ -103,7 +113,7 @@ class CollisionDetector {
 
 class System {
  public:
-  System() : System(DV_SOLVER_APGD_FUSED) {}
+  System() : System(DV_SOLVER_APGD) {}
   explicit System(int solverType) {
     const int rc = dv_scene_create(&s_, solverType);
     if (rc != DV_OK) throw Error(rc, dv_last_error(nullptr));
@@ -216,6 +226,71 @@ class System {
     return 0;
   }
 
+  // System::importSystem (System.cu:1855-1910): adds the file's rigid bodies
+  // (masses 2600 * 4 * 3.14159 * r^3 / 3 for a sphere, 1 for a box), owned
+  // by this System
+  int importSystem(const std::string& filename) {
+    std::ifstream f(filename.c_str());
+    if (!f) throw Error(DV_ERR_ARG, "cannot open " + filename);
+    std::string line;
+    std::getline(f, line);
+    for (char& c : line)
+      if (c == ',') c = ' ';
+    int numBodies = 0;
+    std::istringstream(line) >> numBodies;
+    for (int i = 0; i < numBodies; ++i) {
+      if (!std::getline(f, line)) throw Error(DV_ERR_ARG, "truncated system file " + filename);
+      for (char& c : line)
+        if (c == ',') c = ' ';
+      std::istringstream ss(line);
+      int index = 0, isFixed = 0, shape = 0;
+      double blah = 0;
+      Vec3 pos, vel, geometry;
+      ss >> index >> isFixed >> pos.x >> pos.y >> pos.z >> blah >> blah >> blah >> blah >> vel.x >> vel.y >>
+          vel.z >> shape;
+      if (shape == 0) ss >> geometry.x;
+      else ss >> geometry.x >> geometry.y >> geometry.z;
+      owned_.emplace_back(new Body(pos));
+      Body* b = owned_.back().get();
+      b->setBodyFixed(isFixed != 0);
+      b->setGeometry(geometry);
+      b->setVelocity(vel);
+      b->setMass(shape == 0 ? 2600 * 4.0 * 3.14159 * std::pow(geometry.x, 3.0) / 3.0 : 1.0);
+      add(b);
+    }
+    return 0;
+  }
+
+  // System::exportMatrices (System.cu:1912-1930): D, Minv, r, b and k of the
+  // last step as Matrix Market files in `directory` (coordinate 1-based for
+  // D and Minv, dense columns for the vectors, 17 significant digits)
+  int exportMatrices(const std::string& directory) {
+    size_t nnz = 0;
+    int rows = 0, cols = 0;
+    check(dv_get_jacobian(s_, nullptr, nullptr, nullptr, 0, &nnz, &rows, &cols));
+    std::vector<int> I(nnz ? nnz : 1), J(nnz ? nnz : 1);
+    std::vector<double> V(nnz ? nnz : 1);
+    if (nnz) check(dv_get_jacobian(s_, I.data(), J.data(), V.data(), nnz, &nnz, &rows, &cols));
+    writeCoo(directory + "/D.mtx", I, J, V, nnz, rows, cols);
+    int nb = 0;
+    check(dv_num_bodies(s_, &nb));
+    std::vector<double> mass(nb ? nb : 1);
+    std::vector<int> fixed(nb ? nb : 1);
+    check(dv_get_bodies(s_, mass.data(), nullptr, fixed.data(), nullptr, (size_t)nb));
+    const size_t n3 = 3 * (size_t)nb;
+    std::vector<int> di(n3 ? n3 : 1);
+    std::vector<double> dm(n3 ? n3 : 1);
+    for (size_t q = 0; q < n3; ++q) {
+      di[q] = (int)q;
+      dm[q] = fixed[q / 3] ? 0.0 : 1.0 / mass[q / 3];  // System.cu:380-386
+    }
+    writeCoo(directory + "/Minv.mtx", di, di, dm, n3, (int)n3, (int)n3);
+    writeColumn(directory + "/r.mtx", vec(DV_VEC_R));
+    writeColumn(directory + "/b.mtx", vec(DV_VEC_B));
+    writeColumn(directory + "/k.mtx", vec(DV_VEC_K));
+    return 0;
+  }
+
   dv_scene* handle() { return s_; }
 
  private:
@@ -224,8 +299,24 @@ class System {
     const char* m = dv_last_error(s_);
     throw Error(rc, m ? m : "dvide error");
   }
+  static void writeCoo(const std::string& fn, const std::vector<int>& I, const std::vector<int>& J,
+                       const std::vector<double>& V, size_t nnz, int rows, int cols) {
+    FILE* f = std::fopen(fn.c_str(), "w");
+    if (!f) throw Error(DV_ERR_ARG, "cannot write " + fn);
+    std::fprintf(f, "%%%%MatrixMarket matrix coordinate real general\n%d %d %zu\n", rows, cols, nnz);
+    for (size_t q = 0; q < nnz; ++q) std::fprintf(f, "%d %d %.17g\n", I[q] + 1, J[q] + 1, V[q]);
+    std::fclose(f);
+  }
+  static void writeColumn(const std::string& fn, const std::vector<double>& x) {
+    FILE* f = std::fopen(fn.c_str(), "w");
+    if (!f) throw Error(DV_ERR_ARG, "cannot write " + fn);
+    std::fprintf(f, "%%%%MatrixMarket matrix array real general\n%zu 1\n", x.size());
+    for (double v : x) std::fprintf(f, "%.17g\n", v);
+    std::fclose(f);
+  }
   void pushParams() {
-    if (apgd_.antiRelax) throw Error(DV_ERR_UNSUPPORTED, "anti-relaxation is not available on the GPU path");
+    check(dv_set_param(s_, DV_PARAM_APGD_BUILD, apgd_.build));
+    check(dv_set_param(s_, DV_PARAM_ANTI_RELAX, apgd_.antiRelax ? 1 : 0));
     check(dv_set_param(s_, DV_PARAM_H, h));
     check(dv_set_param(s_, DV_PARAM_MU, mu));
     check(dv_set_param(s_, DV_PARAM_TOL, solver->tolerance));
@@ -250,6 +341,7 @@ class System {
   }
 
   dv_scene* s_ = nullptr;
+  std::vector<std::unique_ptr<Body>> owned_;  // bodies created by importSystem
   APGD apgd_;
   CollisionDetector cd_;
 };

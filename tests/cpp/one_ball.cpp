@@ -1,19 +1,27 @@
 // A reference-style driver (models/test_oneBall.cu:228-262) on the C++
 // drop-in surface: a ball resting on a fixed ground box, stepped, with its
-// normal impulse and contact force printed for the test to check
-// (gamma_n = m g h, f_contact = m g).
+// normal impulse, contact force and CCP violation printed for the test to
+// check (gamma_n = m g h, f_contact = m g); then System::exportSystem,
+// System::exportMatrices into argv[2], a second System built by
+// System::importSystem from the exported file and stepped once, and the
+// factory's answer to a solver type that is not on this path (2 = PDIP).
 #include <cstdio>
+#include <cstdlib>
+#include <string>
 
 #include "dvide/System.hpp"
 
+using dvide::APGD;
 using dvide::Body;
 using dvide::System;
 using dvide::make_vec3;
 
 int main(int argc, char** argv) {
-  const int solverType = argc > 1 ? std::atoi(argv[1]) : DV_SOLVER_APGD_FUSED;
+  const int build = argc > 1 ? std::atoi(argv[1]) : DV_APGD_FP64;
+  const std::string dir = argc > 2 ? argv[2] : ".";
   try {
-    System* sys = new System(solverType);
+    System* sys = new System(1);
+    dynamic_cast<APGD*>(sys->solver)->setBuild(build);
     sys->setTimeStep(1e-2);
     sys->solver->tolerance = 1e-8;
     sys->solver->maxIterations = 10000;
@@ -30,15 +38,37 @@ int main(int argc, char** argv) {
     std::vector<double> g = sys->gamma(), f = sys->f_contact(), v = sys->v();
     dvide::Vec4 viol = sys->getCCPViolation();
     std::printf("collisions %d iterations %d gamma %.17g %.17g %.17g f_contact_y %.17g v_y %.17g time %.17g "
-                "violation %.3g\n",
+                "violation_x %.17g violation_y %.17g violation_z %.17g violation_w %.17g\n",
                 sys->collisionDetector->numCollisions, sys->solver->iterations, g[0], g[1], g[2], f[4], v[4],
-                sys->getCurrentTime(), viol.w);
+                sys->getCurrentTime(), viol.x, viol.y, viol.z, viol.w);
+    sys->exportSystem(dir + "/system.csv");
+    sys->exportMatrices(dir);
     delete sys;
     delete ball;
     delete ground;
+
+    System* again = new System();
+    dynamic_cast<APGD*>(again->solver)->setBuild(build);
+    again->setTimeStep(1e-2);
+    again->solver->tolerance = 1e-8;
+    again->solver->maxIterations = 10000;
+    again->importSystem(dir + "/system.csv");
+    again->initializeSystem();
+    again->DoTimeStep();
+    std::vector<double> g2 = again->gamma();
+    std::printf("imported_bodies %zu imported_mass %.17g imported_collisions %d imported_gamma %.17g\n",
+                again->bodies.size(), again->bodies[1]->getMass(), again->collisionDetector->numCollisions,
+                g2[0]);
+    delete again;
   } catch (const dvide::Error& e) {
     std::fprintf(stderr, "error %d: %s\n", e.code, e.what());
     return 2;
+  }
+  try {
+    System pdip(2);
+    std::printf("factory_pdip accepted\n");
+  } catch (const dvide::Error& e) {
+    std::printf("factory_pdip %d\n", e.code);
   }
   return 0;
 }

@@ -50,7 +50,7 @@ def test_no_gpu_means_an_error_not_a_fallback():
         "import sys; sys.path.insert(0, %r)\n"
         "from dvide_amd import system\n"
         "try:\n"
-        "    system.System(system.SOLVER_FUSED)\n"
+        "    system.System()\n"
         "except system.DvideError as e:\n"
         "    print('ERR', e.code); sys.exit(0)\n"
         "sys.exit(3)\n" % ROOT
@@ -58,6 +58,23 @@ def test_no_gpu_means_an_error_not_a_fallback():
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ERR -3" in r.stdout
+
+
+def test_factory_rejects_solvers_off_this_path():
+    """System(int solverType), System.cu:77-101: 2..7 are PDIP, TPAS, JKIP,
+    PGJ, PGS and PJKIP; the library refuses them before it looks for a
+    device, so the answer is the same with or without a GPU."""
+    lib = system.load_library()
+    for t in range(2, 8):
+        h = ctypes.c_void_p()
+        assert lib.dv_scene_create(ctypes.byref(h), t) == system.DV_ERR_UNSUPPORTED
+        assert not h.value
+        msg = lib.dv_last_error(None).decode()
+        assert ["PDIP", "TPAS", "JKIP", "PGJ", "PGS", "PJKIP"][t - 2] in msg
+    if not system.gpu_visible():
+        for t in (0, 1, 8, -1):  # APGD (1) and the reference's default case
+            h = ctypes.c_void_p()
+            assert lib.dv_scene_create(ctypes.byref(h), t) == -3  # DV_ERR_HIP: no device, not refused
 
 
 def test_scene_generators_are_deterministic():

@@ -45,7 +45,7 @@ def systems(solver, tol=1e-5):
     return [system.from_scene(make(*c), solver, tol=tol, max_iterations=5000) for c in CASES]
 
 
-@pytest.mark.parametrize("solver", [system.SOLVER_FUSED, system.SOLVER_MIXED])
+@pytest.mark.parametrize("solver", [system.BUILD_FP64, system.BUILD_MIXED])
 def test_batch_equals_alone(gpu, solver):
     alone = systems(solver)
     together = systems(solver)
@@ -67,7 +67,7 @@ def test_batch_equals_alone(gpu, solver):
 def test_batch_with_a_scene_without_contacts(gpu):
     """A member with nothing to solve (free fall) is only updated, and the
     others are solved as alone."""
-    solver = system.SOLVER_MIXED
+    solver = system.BUILD_MIXED
 
     def members():
         return [system.from_scene(make("pile", 31, 900), solver, tol=1e-5, max_iterations=3000),
@@ -91,7 +91,7 @@ def test_batch_with_a_scene_without_contacts(gpu):
 def test_batch_of_many_equals_alone(gpu):
     """A batch large enough that its scenes' collision detection and
     assembly are queued from several host threads gives the same results."""
-    solver = system.SOLVER_MIXED
+    solver = system.BUILD_MIXED
     cases = [("pile", 100 + i, 200 + 37 * i) for i in range(40)]
     alone = [system.from_scene(make(*c), solver, tol=1e-5, max_iterations=3000) for c in cases]
     together = [system.from_scene(make(*c), solver, tol=1e-5, max_iterations=3000) for c in cases]
@@ -111,7 +111,7 @@ def test_batch_of_many_equals_alone(gpu):
 def test_server_mix_equals_alone(gpu, max_slots, budget):
     """Slots fewer than requests and a budget that admits one scene at a time
     change only when each request runs, not its result."""
-    solver = system.SOLVER_MIXED
+    solver = system.BUILD_MIXED
     alone = systems(solver)
     served = systems(solver)
     srv = system.Server(solver, max_slots=max_slots, hbm_budget=budget)
@@ -129,7 +129,7 @@ def test_server_mix_equals_alone(gpu, max_slots, budget):
 def test_server_request_without_contacts(gpu):
     """A request with nothing to solve completes at admission, beside
     requests that are solved."""
-    solver = system.SOLVER_MIXED
+    solver = system.BUILD_MIXED
 
     def members():
         return [system.from_scene(make("pile", 41, 700), solver, tol=1e-5, max_iterations=3000),
@@ -151,7 +151,7 @@ def test_server_request_without_contacts(gpu):
 
 def test_server_staggered_arrivals(gpu):
     """Requests submitted while others iterate join at chunk boundaries."""
-    solver = system.SOLVER_FUSED
+    solver = system.BUILD_FP64
     alone = systems(solver)
     served = systems(solver)
     for g in alone:
@@ -169,8 +169,8 @@ def test_server_staggered_arrivals(gpu):
 
 
 def test_server_rejects_a_second_request_in_flight(gpu):
-    g = system.from_scene(make("pile", 21, 3000), system.SOLVER_MIXED)
-    srv = system.Server(system.SOLVER_MIXED, max_slots=2)
+    g = system.from_scene(make("pile", 21, 3000), system.BUILD_MIXED)
+    srv = system.Server(system.BUILD_MIXED, max_slots=2)
     srv.submit(g)
     with pytest.raises(system.DvideError):
         srv.submit(g)
@@ -180,4 +180,4 @@ def test_server_rejects_a_second_request_in_flight(gpu):
 
 def test_batch_rejects_the_plain_solver(gpu):
     with pytest.raises(system.DvideError):
-        system.Batch(system.SOLVER_PLAIN)
+        system.Batch(system.BUILD_PLAIN)

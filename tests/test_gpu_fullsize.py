@@ -52,7 +52,7 @@ def test_full_size_contacts_assembly_and_product(gpu, workload):
     g.close()
 
 
-@pytest.mark.parametrize("solver", [system.SOLVER_FUSED, system.SOLVER_MIXED])
+@pytest.mark.parametrize("solver", [system.BUILD_FP64, system.BUILD_MIXED])
 def test_full_size_step(gpu, workload, solver):
     o, g = both(workload, solver=solver, tol=TOL, max_iterations=20000)
     g.step()
@@ -62,7 +62,7 @@ def test_full_size_step(gpu, workload, solver):
     o.assemble()  # the oracle's D, r at the same (initial) state
     gamma = g.gamma
     res_o = o.residual_of(gamma)
-    if solver == system.SOLVER_FUSED:
+    if solver == system.BUILD_FP64:
         assert abs(res_o - st.residual) <= 1e-3 * TOL, (res_o, st.residual)
     else:
         assert res_o < 1.1 * TOL, res_o

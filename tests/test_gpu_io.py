@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 def test_exported_jacobian_and_vectors_equal_oracle(gpu, tmp_path):
     sc = scenes.cloud(31, n=400, edge=0.6)
-    o, g = both(sc, solver=system.SOLVER_PLAIN)
+    o, g = both(sc, solver=system.BUILD_PLAIN)
     o.step()
     g.step()
     io.export_matrices(g, str(tmp_path))
@@ -39,7 +39,7 @@ def test_exported_jacobian_and_vectors_equal_oracle(gpu, tmp_path):
 
 def test_contact_forces_per_collision(gpu):
     sc = scenes.pile(33, 1500)
-    o, g = both(sc, solver=system.SOLVER_PLAIN)
+    o, g = both(sc, solver=system.BUILD_PLAIN)
     o.step()
     g.step()
     normal, friction = g.contactForcePerCollision()
@@ -62,11 +62,11 @@ def test_contact_forces_per_collision(gpu):
 
 def test_export_import_round_trip(gpu, tmp_path):
     sc = scenes.pile(34, 1200)
-    g = system.from_scene(sc, system.SOLVER_MIXED)
+    g = system.from_scene(sc, system.BUILD_MIXED)
     g.step()
     f = tmp_path / "sys.csv"
     io.export_system(g, str(f), precision=17)
-    h = system.System(system.SOLVER_MIXED)
+    h = system.System(system.BUILD_MIXED)
     assert io.import_system(h, str(f)) == sc.n
     h.initializeSystem()
     assert np.array_equal(h.p, g.p) and np.array_equal(h.v, g.v)
@@ -83,7 +83,7 @@ def test_export_import_round_trip(gpu, tmp_path):
 
 def test_energies(gpu):
     sc = scenes.free_fall(4, n=5)
-    g = system.from_scene(sc, system.SOLVER_FUSED)
+    g = system.from_scene(sc, system.BUILD_FP64)
     g.step()
     p, v = g.p.reshape(-1, 3), g.v.reshape(-1, 3)
     grav = np.asarray(sc.params["gravity"])

@@ -18,11 +18,11 @@ from dvide_amd import scenes, system
 
 pytestmark = pytest.mark.gpu
 
-SOLVERS = [system.SOLVER_PLAIN, system.SOLVER_FUSED, system.SOLVER_MIXED]
+SOLVERS = [system.BUILD_PLAIN, system.BUILD_FP64, system.BUILD_MIXED]
 NAMES = ["plain", "fused", "mixed"]
 # the mixed solver rounds D, M^-1 and the products to binary32 and is run to
 # a residual of 1e-6 here; the FP64 solvers to the scene's 1e-8
-TOL = {system.SOLVER_PLAIN: 1e-8, system.SOLVER_FUSED: 1e-8, system.SOLVER_MIXED: 1e-6}
+TOL = {system.BUILD_PLAIN: 1e-8, system.BUILD_FP64: 1e-8, system.BUILD_MIXED: 1e-6}
 
 
 @pytest.mark.parametrize("solver", SOLVERS, ids=NAMES)
@@ -37,7 +37,7 @@ def test_ball_on_box_stick_slip(gpu, solver, ratio, sticks):
     s.DoTimeStep()
     assert s.numCollisions == 1
     vx, vy, vz = s.v[3:6]
-    lim = 1e-7 if solver != system.SOLVER_MIXED else 1e-5
+    lim = 1e-7 if solver != system.BUILD_MIXED else 1e-5
     assert abs(vz) < 1e-12
     if sticks:
         assert abs(vx) < lim and abs(vy) < lim, (vx, vy)
@@ -51,7 +51,7 @@ def test_ball_on_box_stick_slip(gpu, solver, ratio, sticks):
 @pytest.mark.parametrize("solver", SOLVERS, ids=NAMES)
 def test_momentum_identities_in_a_pile(gpu, solver):
     sc = scenes.pile(5, 600)
-    s = system.from_scene(sc, solver, tol=1e-6 if solver != system.SOLVER_MIXED else 1e-5,
+    s = system.from_scene(sc, solver, tol=1e-6 if solver != system.BUILD_MIXED else 1e-5,
                           max_iterations=20000)
     v0 = s.v.copy()
     s.DoTimeStep()

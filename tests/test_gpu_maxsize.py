@@ -45,7 +45,7 @@ def test_max_size_contacts_and_r(gpu, workload):
 
 
 def test_max_size_mixed_step(gpu, workload):
-    o, g = both(workload, solver=system.SOLVER_MIXED, tol=1e-4, max_iterations=3000)
+    o, g = both(workload, solver=system.BUILD_MIXED, tol=1e-4, max_iterations=3000)
     g.step()
     st = g.stats()
     assert st.num_contacts > 900000 and 0 < st.iterations <= 3000

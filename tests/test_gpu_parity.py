@@ -91,7 +91,7 @@ def test_schur_product_bit_exact(gpu):
 def test_plain_trajectory_bit_exact(gpu):
     """200-step lattice drop: the plain FP64 path tracks the oracle bit for bit."""
     sc = scenes.lattice_box(1001)
-    o, g = both(sc, solver=system.SOLVER_PLAIN)
+    o, g = both(sc, solver=system.BUILD_PLAIN)
     for k in range(200):
         o.step()
         g.step()
@@ -109,7 +109,7 @@ def test_fp64_solver_matches_oracle(gpu):
     products are summed in another order, so the trajectories agree to
     rounding and the solutions to well below the tolerance."""
     sc = scenes.compaction(2002, n=4000, width=1.5, lid=scenes.LID_BILATERAL)
-    o, g = both(sc, solver=system.SOLVER_FUSED, tol=1e-6, max_iterations=20000)
+    o, g = both(sc, solver=system.BUILD_FP64, tol=1e-6, max_iterations=20000)
     o.step()
     g.step()
     st = g.stats()
@@ -133,7 +133,7 @@ def test_mixed_solver_early_iterations_track_oracle(gpu):
     the binary32 rounding of D, M^-1 and the products."""
     sc = scenes.compaction(2002, n=4000, width=1.5, lid=scenes.LID_BILATERAL)
     for m in (1, 5, 10):
-        o, g = both(sc, solver=system.SOLVER_MIXED, tol=0.0, max_iterations=m)
+        o, g = both(sc, solver=system.BUILD_MIXED, tol=0.0, max_iterations=m)
         o.step()
         g.step()
         assert g.stats().iterations == o.iterations == m
@@ -154,7 +154,7 @@ def test_mixed_solver_at_matched_residual(gpu):
     exact = oracle.OracleSystem(sc, tol=1e-8, max_iterations=100000)
     exact.step()
     assert exact.residual < 1e-7
-    o, g = both(sc, solver=system.SOLVER_MIXED, tol=tol, max_iterations=20000)
+    o, g = both(sc, solver=system.BUILD_MIXED, tol=tol, max_iterations=20000)
     o.step()
     g.step()
     st = g.stats()
@@ -171,17 +171,17 @@ def test_mixed_solver_at_matched_residual(gpu):
 
 
 def test_ball_on_box_gpu(gpu):
-    for solver in (system.SOLVER_PLAIN, system.SOLVER_FUSED, system.SOLVER_MIXED):
+    for solver in (system.BUILD_PLAIN, system.BUILD_FP64, system.BUILD_MIXED):
         g = system.from_scene(scenes.ball_on_box(0.0), solver)
         g.DoTimeStep()
         gam = g.gamma
-        assert abs(gam[0] - 0.0981) < 1e-6 * (1e3 if solver == system.SOLVER_MIXED else 1), (solver, gam)
-        assert abs(g.f_contact[4] - 9.81) < 1e-4 * (1e2 if solver == system.SOLVER_MIXED else 1)
+        assert abs(gam[0] - 0.0981) < 1e-6 * (1e3 if solver == system.BUILD_MIXED else 1), (solver, gam)
+        assert abs(g.f_contact[4] - 9.81) < 1e-4 * (1e2 if solver == system.BUILD_MIXED else 1)
 
 
 def test_free_fall_gpu(gpu):
     sc = scenes.free_fall(5, n=8)
-    g = system.from_scene(sc, system.SOLVER_FUSED)
+    g = system.from_scene(sc, system.BUILD_FP64)
     p0, v0 = g.p.copy(), g.v.copy()
     g.DoTimeStep()
     assert g.numCollisions == 0
@@ -191,7 +191,7 @@ def test_free_fall_gpu(gpu):
     assert np.allclose(g.p, p0 + h * g.v, rtol=0, atol=1e-15)
 
 
-@pytest.mark.parametrize("solver,bound", [(system.SOLVER_FUSED, 1e-10), (system.SOLVER_MIXED, 1e-5)])
+@pytest.mark.parametrize("solver,bound", [(system.BUILD_FP64, 1e-10), (system.BUILD_MIXED, 1e-5)])
 def test_rows_per_thread_agree(gpu, solver, bound):
     """One or two contacts per thread in the contact pass P1 only change the
     order of the partial sums: the same ten iterations agree to rounding."""
@@ -221,7 +221,7 @@ def test_fp64_solver_velocities_match_pgs(gpu):
     finally:
         oracle.set_threads(prev)
     assert p.residual < 1e-10
-    g = system.from_scene(sc, system.SOLVER_FUSED, tol=1e-8, max_iterations=100000)
+    g = system.from_scene(sc, system.BUILD_FP64, tol=1e-8, max_iterations=100000)
     g.step()
     assert g.stats().residual < 1e-8
     free = np.repeat(sc.fixed == 0, 3)

@@ -15,23 +15,7 @@ import pytest
 import oracle
 from dvide_amd import scenes
 
-from helpers import pair_keys
-
-
-def tiny_scene(pos, geom, mass=None, fixed=None, family=None, vel=None, gravity=(0.0, -9.81, 0.0),
-               h=1e-3, mu=0.25, tol=1e-8, max_iterations=20000, envelope=1e-3):
-    n = len(pos)
-    sc = scenes.Scene(
-        pos=np.asarray(pos, np.float64).reshape(n, 3),
-        vel=np.zeros((n, 3)) if vel is None else np.asarray(vel, np.float64).reshape(n, 3),
-        mass=np.ones(n) if mass is None else np.asarray(mass, np.float64),
-        geom=np.asarray(geom, np.float64).reshape(n, 3),
-        fixed=np.zeros(n, np.int32) if fixed is None else np.asarray(fixed, np.int32),
-        family=-np.ones(n, np.int32) if family is None else np.asarray(family, np.int32),
-        bil_dofs=np.zeros((0, 2), np.int32), bil_info=np.zeros((0, 2)))
-    sc.params = dict(h=h, mu=mu, tol=tol, max_iterations=max_iterations, gravity=gravity,
-                     envelope=envelope, bins=20, steps=1, warm_start=False)
-    return sc
+from helpers import pair_keys, tiny_scene
 
 
 # --- narrowphase: closed forms ------------------------------------------------
