@@ -191,10 +191,21 @@ def _gen_many(specs):
         return list(ex.map(lambda a: scenes.pile(*a), specs))
 
 
-def batched(n_scenes=512, steps=2, warmup=1, seed=4004):
+# iteration caps of the batched and served legs: the deposit transient of a
+# fresh pile is settled under SETTLE_CAP, and every timed solve runs to the
+# tolerance under the reference's default cap (APGD.cu:11)
+SETTLE_CAP = 20000
+REFERENCE_CAP = 100000
+SERVE_SETTLE = 6
+
+
+def batched(n_scenes=512, steps=2, settle=3, seed=4004):
     """512 independent piles, contacts log-uniform in [1k, 100k], radii
     U[0.03, 0.06] m, mu U[0.1, 0.5]; one mixed-precision APGD solve of all of
-    them per step in the same graph launches."""
+    them per step in the same graph launches.  The fresh deposits are first
+    settled for `settle` untimed batch steps; the timed steps solve every
+    scene to the tolerance under the reference's iteration cap, and the
+    converged count and the largest residual are reported beside the rate."""
     from dvide_amd import system
 
     rng = np.random.default_rng(seed)
@@ -203,32 +214,44 @@ def batched(n_scenes=512, steps=2, warmup=1, seed=4004):
     t0 = time.perf_counter()
     scs = _gen_many([(seed + i, int(targets[i]), 0.03, 0.06, float(mus[i])) for i in range(n_scenes)])
     gen = time.perf_counter() - t0
-    systems = [system.from_scene(sc, system.BUILD_MIXED, tol=TOL) for sc in scs]
+    systems = [system.from_scene(sc, system.BUILD_MIXED, tol=TOL, max_iterations=SETTLE_CAP) for sc in scs]
     batch = system.Batch(system.BUILD_MIXED)
     for g in systems:
         batch.add(g)
-    for _ in range(warmup):
+    t0 = time.perf_counter()
+    for _ in range(settle):
         batch.step()
+    settle_s = time.perf_counter() - t0
+    for g in systems:
+        g.solver.maxIterations = REFERENCE_CAP
     hip = _hip()
     hip.hipDeviceSynchronize()
     t0 = time.perf_counter()
-    stats = []
+    stats, res = [], []
     for _ in range(steps):
         batch.step()
         stats.append(batch.stats())
+        res.append([g.stats().residual for g in systems if g.stats().num_contacts])
     hip.hipDeviceSynchronize()
     dt = time.perf_counter() - t0
     contacts = np.mean([s.total_contacts for s in stats])
+    res_all = np.concatenate([np.asarray(r) for r in res])
     out = {
         "scenes": n_scenes,
         "scene_steps_per_s": round(n_scenes * steps / dt, 2),
         "contact_steps_per_s": round(contacts * steps / dt, 1),
         "ms_per_batch_step": round(1e3 * dt / steps, 2),
         "contacts_per_step": int(contacts),
+        "scene_steps_solved": int(res_all.size),
+        "scene_steps_converged": int(np.sum(res_all < TOL)),
+        "residual_max": float(res_all.max()),
+        "iteration_cap": REFERENCE_CAP,
         "max_iterations": int(max(s.max_iterations for s in stats)),
         "mean_iterations": round(float(np.mean([s.total_iterations / n_scenes for s in stats])), 1),
         "ms_assembly": round(float(np.mean([s.ms_assembly for s in stats])), 2),
         "ms_solve": round(float(np.mean([s.ms_solve for s in stats])), 2),
+        "settle_batch_steps": settle,
+        "settle_s": round(settle_s, 1),
         "scene_generation_s": round(gen, 1),
     }
     batch.close()
@@ -247,8 +270,17 @@ def served(n_scenes=256, n_requests=512, load=0.7, seed=5005, max_slots=64):
     rng = np.random.default_rng(seed)
     targets = np.exp(rng.uniform(np.log(1e3), np.log(2e5), n_scenes)).astype(int)
     scs = _gen_many([(seed + i, int(targets[i]), 0.02, 0.03, 0.4) for i in range(n_scenes)])
-    systems = [system.from_scene(sc, system.BUILD_MIXED, tol=TOL) for sc in scs]
+    systems = [system.from_scene(sc, system.BUILD_MIXED, tol=TOL, max_iterations=SETTLE_CAP) for sc in scs]
     srv = system.Server(system.BUILD_MIXED, max_slots=max_slots)
+    # the fresh deposits' transient: untimed requests, every scene in turn
+    t0 = time.perf_counter()
+    for _ in range(SERVE_SETTLE):
+        for g in systems:
+            srv.submit(g)
+        srv.drain()
+    settle_s = time.perf_counter() - t0
+    for g in systems:
+        g.solver.maxIterations = REFERENCE_CAP
     # closed loop: every scene once, as fast as the server completes them
     t0 = time.perf_counter()
     for g in systems:
@@ -261,7 +293,7 @@ def served(n_scenes=256, n_requests=512, load=0.7, seed=5005, max_slots=64):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
     which = rng.integers(0, n_scenes, n_requests)
     busy, queued, ticket_of = set(), {}, {}
-    submitted_at, latency = {}, []
+    submitted_at, latency, residuals = {}, [], []
     start = time.perf_counter()
     nxt = 0
     while len(latency) < n_requests:
@@ -281,7 +313,11 @@ def served(n_scenes=256, n_requests=512, load=0.7, seed=5005, max_slots=64):
         if srv.pending():
             for tk in srv.poll():
                 latency.append(time.perf_counter() - start - submitted_at.pop(tk))
-                busy.discard(ticket_of.pop(tk))
+                sc_i = ticket_of.pop(tk)
+                busy.discard(sc_i)
+                st = systems[sc_i].stats()
+                if st.num_contacts:
+                    residuals.append(st.residual)
         elif nxt < n_requests:
             time.sleep(max(0.0, min(0.001, arrivals[nxt] - now)))
     total = time.perf_counter() - start
@@ -298,29 +334,170 @@ def served(n_scenes=256, n_requests=512, load=0.7, seed=5005, max_slots=64):
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
                        "p99": round(float(np.percentile(lat, 99)), 2),
                        "mean": round(float(lat.mean()), 2)},
+        "settle_requests_per_scene": SERVE_SETTLE,
+        "settle_s": round(settle_s, 1),
+        "requests_solved": len(residuals),
+        "requests_converged": int(np.sum(np.asarray(residuals) < TOL)),
+        "residual_max": float(max(residuals)) if residuals else None,
+        "iteration_cap": REFERENCE_CAP,
         "max_slots": max_slots,
         "contacts_mean": int(np.mean(targets)),
     }
 
 
-def cpu_baseline(scene, steps):
-    """The oracle (FP64 C++/OpenMP restatement) on the host cores, from the
-    settled state: one untimed step fills its warm-start gamma."""
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def host_cores():
+    """The host cores this process may use: the affinity mask, within
+    OMP_NUM_THREADS when that is set."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
+
+
+def cpu_baseline(scene, steps, threads, warm_fill=True):
+    """The oracle (FP64 C++/OpenMP restatement) on `threads` host cores,
+    from the settled state: with `warm_fill`, one untimed step first fills
+    its warm-start gamma; then `steps` timed steps.  Returns the figures and
+    the oracle's state after its first step, which the error check below
+    compares the GPU solvers with."""
     import oracle
 
-    cores = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
-    oracle.set_threads(cores)
+    oracle.set_threads(threads)
     o = oracle.OracleSystem(scene, tol=TOL)
-    o.step()
+    first = None
+    if warm_fill:
+        o.step()
+        first = dict(v=o.vec(oracle.V), f_contact=o.vec(oracle.F_CONTACT), gamma=o.vec(oracle.GAMMA),
+                     iterations=o.iterations, residual=o.residual)
     its = []
     t0 = time.perf_counter()
     for _ in range(steps):
         o.step()
         its.append(o.iterations)
     dt = time.perf_counter() - t0
-    return dict(value=steps / dt, unit="time-steps/s", cores=cores, kind="port",
-                sample=f"{steps} time steps of the same workload from the settled state (after one untimed "
-                       f"step), APGD to tol {TOL} ({'/'.join(map(str, its))} iterations), FP64, {dt:.1f} s")
+    start = "after one untimed step" if warm_fill else "its first step, gamma from zero"
+    out = dict(value=steps / dt, unit="time-steps/s", cores=threads, kind="port",
+               sample=f"{steps} time step(s) of the same workload from the settled state ({start}), "
+                      f"APGD to tol {TOL} ({'/'.join(map(str, its))} iterations), FP64, "
+                      f"{threads} thread(s), {dt:.1f} s")
+    return out, first
+
+
+def oracle_threads(n):
+    import oracle
+
+    oracle.set_threads(n)
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _one_step_errors(scene, oracle_first, builds, converged=None):
+    """One step of each GPU build from `scene` against the oracle's step from
+    the same state at the same tolerance (FP64): relative errors of the free
+    bodies' velocities and contact forces, the gamma error, and the residual
+    the oracle evaluates in FP64 on the GPU's gamma.  With `converged` (the
+    oracle's step at a tight tolerance), also each solution's velocity error
+    against it, the oracle's own at the same tolerance included: at a
+    residual of 1e-4 two solvers can stop at different points, and the
+    distance to the converged solution says how far each is."""
+    import oracle
+    from dvide_amd import system
+
+    free = np.repeat(np.asarray(scene.fixed) == 0, 3)
+    o_asm = oracle.OracleSystem(scene, tol=TOL)
+    o_asm.assemble()
+    out = {}
+    for name, build in builds:
+        g = system.from_scene(scene, build, tol=TOL, max_iterations=REFERENCE_CAP)
+        g.step()
+        st = g.stats()
+        out[name] = {
+            "contacts": st.num_contacts,
+            "iterations": st.iterations,
+            "residual": float(f"{st.residual:.3g}"),
+            "residual_by_oracle": float(f"{o_asm.residual_of(g.gamma):.3g}"),
+            "rel_v": float(f"{_rel(g.v[free], oracle_first['v'][free]):.3g}"),
+            "rel_f_contact": float(f"{_rel(g.f_contact[free], oracle_first['f_contact'][free]):.3g}"),
+            "rel_gamma": float(f"{_rel(g.gamma, oracle_first['gamma']):.3g}"),
+        }
+        if converged is not None:
+            out[name]["rel_v_vs_converged"] = float(f"{_rel(g.v[free], converged['v'][free]):.3g}")
+        g.close()
+    if converged is not None:
+        out["oracle_rel_v_vs_converged"] = float(f"{_rel(oracle_first['v'][free], converged['v'][free]):.3g}")
+        out["converged_tol"] = converged["tol"]
+        out["converged_residual"] = float(f"{converged['residual']:.3g}")
+    return out
+
+
+def _oracle_first_step(scene, tol=TOL, max_iterations=REFERENCE_CAP):
+    import oracle
+
+    o = oracle.OracleSystem(scene, tol=tol, max_iterations=max_iterations)
+    o.step()
+    return dict(v=o.vec(oracle.V), f_contact=o.vec(oracle.F_CONTACT), gamma=o.vec(oracle.GAMMA),
+                iterations=o.iterations, residual=o.residual, tol=tol)
+
+
+def error_check(headline_scene, headline_first):
+    """The mixed solver against the FP64 oracle, per workload, at matched
+    residual (both solved to tol 1e-4 from the same state); the FP64
+    graph-driven build beside it shows how far two FP64 solutions at that
+    tolerance lie apart.  gamma is unique only on the single-layer scene
+    (one contact per sphere); on piles N is rank-deficient, so v and
+    f_contact carry the comparison there."""
+    import oracle
+    from dvide_amd import scenes, system
+
+    builds = (("mixed", system.BUILD_MIXED), ("fp64", system.BUILD_FP64))
+    rep = {"tol": TOL}
+    rep["compaction_112k"] = _one_step_errors(headline_scene, headline_first, builds)
+    # the 1,000-sphere lattice drop, in contact after 100 steps of the oracle
+    lat = scenes.lattice_box(1001)
+    o = oracle.OracleSystem(lat)
+    for _ in range(100):
+        o.step()
+    lat.pos = o.vec(oracle.P).reshape(-1, 3).copy()
+    lat.vel = o.vec(oracle.V).reshape(-1, 3).copy()
+    rep["lattice_1000_step_101"] = _one_step_errors(lat, _oracle_first_step(lat), builds,
+                                                    _oracle_first_step(lat, tol=1e-7, max_iterations=30000))
+    # four members of the 512-scene batch family (seed 4004 + i), first step
+    rng = np.random.default_rng(4004)
+    targets = np.exp(rng.uniform(np.log(1e3), np.log(1e5), 512)).astype(int)
+    mus = rng.uniform(0.1, 0.5, 512)
+    fam = {}
+    for i in (0, 1, 2, 3):
+        sc = scenes.pile(4004 + i, int(targets[i]), 0.03, 0.06, float(mus[i]))
+        fam[f"member_{i}"] = _one_step_errors(sc, _oracle_first_step(sc), builds,
+                                              _oracle_first_step(sc, tol=1e-6, max_iterations=8000))
+    rep["batch_members"] = fam
+    # gamma on the single layer (unique), at 1e-4 and against the converged 1e-8 solution
+    lay = scenes.layer(7)
+    rep["layer_256_gamma_unique"] = _one_step_errors(lay, _oracle_first_step(lay), builds)
+    tight = _oracle_first_step(lay, tol=1e-8)
+    g = system.from_scene(lay, system.BUILD_MIXED, tol=TOL, max_iterations=REFERENCE_CAP)
+    g.step()
+    rep["layer_256_gamma_unique"]["mixed"]["rel_gamma_vs_converged"] = float(
+        f"{_rel(g.gamma, tight['gamma']):.3g}")
+    g.close()
+    return rep
+
+
+def progress(msg):
+    """A progress line on stderr (the result line alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -354,7 +531,9 @@ def main():
 
     from dvide_amd import scenes, system
 
+    progress("settling the compaction pile")
     scene, settle_its = settled_scene(args.settle)
+    progress("mixed solver")
     cap = args.max_iterations
     mixed = run_gpu(scene, system.BUILD_MIXED, args.steps, args.warmup, profile_kernel=system.PROFILE_P1,
                     max_iterations=cap)
@@ -367,6 +546,7 @@ def main():
             kernel_us[name] = round(1e3 * r["kernel_ms"] / r["kernel_count"], 3)
     others = {}
     if not args.only_mixed:
+        progress("FP64 graph-driven and plain FP64 solvers")
         others["fp64"] = run_gpu(scene, system.BUILD_FP64, args.steps, args.warmup, max_iterations=cap)
         others["plain_fp64"] = run_gpu(scene, system.BUILD_PLAIN, max(2, args.steps // 4), args.warmup and 1,
                                        max_iterations=cap)
@@ -476,13 +656,29 @@ def main():
         }
     line["settle_iterations"] = settle_its
     if not args.only_mixed:
+        progress("lid-pressed leg")
         line["lid_pressed"] = lid_pressed()
     if not args.no_batch and not args.only_mixed:
+        progress("512-scene batch")
         line["batched"] = batched()
     if not args.no_serve and not args.only_mixed:
+        progress("served stream")
         line["served"] = served()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(scene, args.cpu_steps)
+        # the CPU baseline leg: the oracle timed on the host cores (all of this
+        # process's, then one), and, as the checker, the GPU solvers'
+        # per-workload errors against it
+        cores = host_cores()
+        progress(f"CPU baseline on {cores} cores")
+        base, first = cpu_baseline(scene, args.cpu_steps, cores)
+        progress("CPU baseline on 1 core")
+        one, _ = cpu_baseline(scene, 1, 1, warm_fill=False)
+        base["cpu_model"] = cpu_model()
+        base["single_thread"] = {"value": one["value"], "cores": 1, "sample": one["sample"]}
+        line["cpu_baseline"] = base
+        progress("error check against the oracle")
+        oracle_threads(cores)
+        line["error_vs_fp64_oracle"] = error_check(scene, first)
     print(json.dumps(line), flush=True)
 
 

@@ -59,7 +59,13 @@
 #include "dv_internal.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+#include <sys/stat.h>
 
 namespace dv {
 
@@ -149,7 +155,8 @@ struct Args {
   const int* cA;
   const int* cB;
   const int* iStart;
-  const int* inc;
+  const int* cStart;    // contacts of body b as body A: [cStart[b], cStart[b+1]) (rows sorted by idA)
+  const int* posB;      // slot of contact c in the B-side order, JB[posB[c]] = J[c]
   const int* bilIncStart;
   const int* bilInc;
   const int* bilA;
@@ -164,7 +171,9 @@ struct Args {
   double* ring[4];      // iterates (component-major), slots named by ApgdState
   TM* rn[4];            // N of the iterate in the same ring slot
   long ringStride;      // ring[k] = ring[0] + k * ringStride, and the same for rn
-  V4<TM>* J;
+  V4<TM>* J;            // per contact, in contact order: body A's incidences are contiguous
+  V4<TM>* JB;           // the same, in B-side order: body b's B incidences are
+                        // [iStart[b] - cStart[b], iStart[b+1] - cStart[b+1])
   V4<TM>* tmp;
   double* gHat;         // gammaHat before any iterate is accepted (ones)
   TM* delta;            // mixed solver: gamma_new - y
@@ -186,6 +195,7 @@ struct Tables {
   PassView* pv;         // [slot][2]: its view for the pass kernels
   ApgdState* out;       // per slot: the state after the latest decision
   int single;           // one slot: tile k is rows k*kTC.. of slot 0 (no table read)
+  int xcd;              // XCD-contiguous tile order for the gathering kernels
 };
 
 // state of slot `slot` at parity k: at a fixed offset from the kernel
@@ -251,31 +261,50 @@ __device__ __forceinline__ double waveMax(double v) {
   return fmax(fmax(laneValue(v, 0), laneValue(v, 16)), fmax(laneValue(v, 32), laneValue(v, 48)));
 }
 
-template <int NT, int NS>
-__device__ void blockReduce(double (&acc)[NS], double* out) {
+// Workgroup totals of NS slots over a tile whose threads hold R rows each:
+// row u * NT + t of the tile is thread t's accumulator acc[u].  Each
+// accumulator is reduced over its wavefront, and the wavefront totals are
+// combined in row order, so the result is that of one row per thread on
+// R * NT threads, bit for bit: the rows-per-thread schedule changes the
+// work per lane, not the arithmetic.
+template <int NT, int R, int NS>
+__device__ void blockReduceR(double (&acc)[R][NS], double* out) {
   constexpr int W = NT / 64;
-  __shared__ double sh[W][NS];
+  __shared__ double sh[R * W][NS];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    double v = (s % kSlots == kSlots - 1) ? waveMax(acc[s]) : waveSum(acc[s]);
-    if (lane == 0) sh[wave][s] = v;
+  for (int u = 0; u < R; ++u) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      double v = (s % kSlots == kSlots - 1) ? waveMax(acc[u][s]) : waveSum(acc[u][s]);
+      if (lane == 0) sh[u * W + wave][s] = v;
+    }
   }
   __syncthreads();
   if (threadIdx.x < NS) {
     const int s = threadIdx.x;
     double v = sh[0][s];
-    for (int w = 1; w < W; ++w) v = (s % kSlots == kSlots - 1) ? fmax(v, sh[w][s]) : v + sh[w][s];
+    for (int w = 1; w < R * W; ++w) v = (s % kSlots == kSlots - 1) ? fmax(v, sh[w][s]) : v + sh[w][s];
     out[s] = v;
   }
   __syncthreads();
 }
 
+template <int NT, int NS>
+__device__ void blockReduce(double (&acc)[NS], double* out) {
+  blockReduceR<NT, 1, NS>(reinterpret_cast<double(&)[1][NS]>(acc), out);
+}
+
+template <int NT, int R = 1>
+__device__ void writePartialsR(double (&acc)[R][kSlots], double* part) {
+  __shared__ double tot[kSlots];
+  blockReduceR<NT, R, kSlots>(acc, tot);
+  if (threadIdx.x < kSlots) gl(part)[threadIdx.x] = tot[threadIdx.x];
+}
+
 template <int NT>
 __device__ void writePartials(double (&acc)[kSlots], double* part) {
-  __shared__ double tot[kSlots];
-  blockReduce<NT, kSlots>(acc, tot);
-  if (threadIdx.x < kSlots) gl(part)[threadIdx.x] = tot[threadIdx.x];
+  writePartialsR<NT, 1>(reinterpret_cast<double(&)[1][kSlots]>(acc), part);
 }
 
 // Fixed-order totals of the per-tile partials of one scene by one wavefront
@@ -410,14 +439,22 @@ __device__ __forceinline__ void ldFrame(const Args<float>& a, int c, int, float 
   F[8] = F[0] * F[4] - F[1] * F[3];
 }
 
+// J_c = F_c^T x_c, stored twice: at c (contact order, where body A's
+// incidences are contiguous) and at posB[c] (B-side order, where body B's
+// are), so that P2 reads each body's incidences as two contiguous runs
 template <typename TM>
-__device__ __forceinline__ void storeJ(const Args<TM>& a, int c, const TM (&F)[9], TM x0, TM x1, TM x2) {
+__device__ __forceinline__ void storeJ(const Args<TM>& a, int c, int cb, const TM (&F)[9], TM x0, TM x1, TM x2) {
   V4<TM> j;
   j.x = F[0] * x0 + F[3] * x1 + F[6] * x2;
   j.y = F[1] * x0 + F[4] * x1 + F[7] * x2;
   j.z = F[2] * x0 + F[5] * x1 + F[8] * x2;
   j.w = 0;
   st4(a.J, c, j);
+  st4(a.JB, cb, j);
+}
+template <typename TM>
+__device__ __forceinline__ void storeJ(const Args<TM>& a, int c, const TM (&F)[9], TM x0, TM x1, TM x2) {
+  storeJ(a, c, gl(a.posB)[c], F, x0, x1, x2);
 }
 
 template <typename TM>
@@ -460,7 +497,10 @@ __device__ __forceinline__ void zero(double (&acc)[kSlots]) {
 }
 
 // loop of a workgroup over the row tiles (uniform across the workgroup)
-#define TILE_LOOP(T) for (int ti_ = blockIdx.x, nt_ = (T).counts[0]; ti_ < nt_; ti_ += gridDim.x)
+#define TILE_LOOP(T) \
+  for (int ti_ = tilesOf(T, (T).counts[0], gridDim.x).first, te_ = tilesOf(T, (T).counts[0], gridDim.x).end, \
+           ts_ = tilesOf(T, (T).counts[0], gridDim.x).step;                                                 \
+       ti_ < te_; ti_ += ts_)
 
 __device__ __forceinline__ Tile rowTile(const Tables& T, int k) {
   return T.single ? Tile{0, k * kTC} : T.items[k];
@@ -471,6 +511,33 @@ __device__ __forceinline__ Tile bodyTile(const Tables& T, int k) {
 
 // partial-sum slots of a tile
 #define TILE_PART(p, tile) ((p) + (long)((tile).start / kTC) * kSlots)
+
+// XCD-contiguous tile order for the gathering kernels of a single slot.  The
+// dispatcher hands workgroup b to XCD b % 8, and each XCD has an L2 of its
+// own: workgroups b = x, x + 8, x + 16, ... take the tiles of the x-th eighth
+// of [0, nt) in order, so that an XCD gathers from the bodies (and contacts)
+// of one contiguous range, whose neighbours it mostly shares lines with,
+// instead of every XCD fetching lines from all over the arrays.  Any grid of
+// at least 8 workgroups covers every tile once (fewer: plain grid stride).
+// Only the order of independent tiles changes, so results do not.
+constexpr int kXcds = 8;
+struct TileRange {
+  int first, end, step;
+};
+__device__ __forceinline__ TileRange xcdTiles(int b, int nw, int nt) {
+  if (nw < kXcds) return TileRange{b, nt, nw};
+  const int x = b % kXcds, j = b / kXcds;
+  const int q = nt / kXcds, r = nt % kXcds;
+  const int start = x * q + min(x, r), cnt = q + (x < r ? 1 : 0);
+  const int wx = nw / kXcds + (x < nw % kXcds ? 1 : 0);  // workgroups on XCD x
+  return TileRange{start + j, start + cnt, wx};
+}
+
+// tiles of this workgroup among nt, by the schedule's order (XCD-contiguous
+// or grid stride) over nw workgroups
+__device__ __forceinline__ TileRange tilesOf(const Tables& T, int nt, int nw) {
+  return T.xcd ? xcdTiles(blockIdx.x, nw, nt) : TileRange{(int)blockIdx.x, nt, nw};
+}
 
 // ---------------------------------------------------------------------------
 // setup (APGD.cu:195-223)
@@ -562,11 +629,17 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
     const bool own = g < a.dims.nb;
     const int gs = own ? g : 0;
     const int k0 = gl(a.iStart)[gs], k1r = gl(a.iStart)[gs + 1];
+    const int c0 = gl(a.cStart)[gs], c1r = gl(a.cStart)[gs + 1];
     const int b0 = gl(a.bilIncStart)[gs], b1r = gl(a.bilIncStart)[gs + 1];
     const TM mi = bodyMinv(a, gs, which, own);
     const bool heavy = k1r - k0 > kHeavyDegree;  // reduced below by a workgroup
     const bool sums = own && !heavy && mi != (TM)0;
-    const int k1 = sums ? k1r : k0, b1 = sums ? b1r : b0;
+    const int b1 = sums ? b1r : b0;
+    // the body's incidences in ascending contact order, as two contiguous
+    // runs: as body B, JB[k0 - c0, k1 - c1) (lower-numbered partners' rows),
+    // then as body A, J[c0, c1) (its own row)
+    const int jb0 = k0 - c0, jb1 = sums ? k1r - c1r : jb0;
+    const int c1 = sums ? c1r : c0;
     const DV_GLOBAL double* X = bodySource(a, which, S);
     const long bil0 = 3L * a.cap;
     V4<TM> out;
@@ -579,22 +652,27 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
         const int comp = (e >> 1) & 3;
         if (comp == 0) s0 += t; else if (comp == 1) s1 += t; else s2 += t;
       }
-      // incidences in batches of 8: all index loads, then all J loads, then
-      // the sums (in incidence order), so the gathers overlap
-      // (slots past the body's last incidence repeat its first one, so that
+      // runs in batches of 8 loads, then the sums in order, so the loads
+      // overlap (slots past a run's end repeat its first entry, so that
       // every load is unconditional)
-      for (int kb = k0; kb < k1; kb += 8) {
-        int e[8];
+      for (int kb = jb0; kb < jb1; kb += 8) {
         V4<TM> j[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) e[u] = gl(a.inc)[kb + u < k1 ? kb + u : kb];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) j[u] = ld4(a.J, e[u] >> 1);
+        for (int u = 0; u < 8; ++u) j[u] = ld4(a.JB, kb + u < jb1 ? kb + u : kb);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          if (kb + u >= k1) continue;
-          if (e[u] & 1) { s0 -= j[u].x; s1 -= j[u].y; s2 -= j[u].z; }
-          else { s0 += j[u].x; s1 += j[u].y; s2 += j[u].z; }
+          if (kb + u >= jb1) continue;
+          s0 -= j[u].x; s1 -= j[u].y; s2 -= j[u].z;
+        }
+      }
+      for (int kb = c0; kb < c1; kb += 8) {
+        V4<TM> j[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) j[u] = ld4(a.J, kb + u < c1 ? kb + u : kb);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          if (kb + u >= c1) continue;
+          s0 += j[u].x; s1 += j[u].y; s2 += j[u].z;
         }
       }
       out.x = mi * s0; out.y = mi * s1; out.z = mi * s2;
@@ -605,10 +683,13 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
     const PassView S = loadView(T, 0, sIdx);
     const Args<TM>& a = A[0];
     const int nt = T.counts[1];
-    if (!(which == 2 && viewDone(S)))
-      for (int ti = blockIdx.x; ti < nt; ti += nw) tileBody(Tile{0, ti * kTB}, S, a);
+    if (!(which == 2 && viewDone(S))) {
+      const TileRange tr = tilesOf(T, nt, nw);
+      for (int ti = tr.first; ti < tr.end; ti += tr.step) tileBody(Tile{0, ti * kTB}, S, a);
+    }
   } else {
-    for (int ti = blockIdx.x, nt = T.counts[1]; ti < nt; ti += nw) {
+    const TileRange tr = tilesOf(T, T.counts[1], nw);
+    for (int ti = tr.first; ti < tr.end; ti += tr.step) {
       const Tile tile = bodyTile(T, ti);
       const PassView S = loadView(T, tile.scene, sIdx);
       if (which == 2 && viewDone(S)) continue;
@@ -633,12 +714,14 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
     const DV_GLOBAL double* X = bodySource(a, which, S);
     const long bil0 = 3L * a.cap;
     const int k0 = gl(a.iStart)[g], k1 = gl(a.iStart)[g + 1];
+    const int c0 = gl(a.cStart)[g], c1 = gl(a.cStart)[g + 1];
+    const int nd = (k1 - k0) - (c1 - c0), jb0 = k0 - c0;  // B-side incidences first
     double acc[kSlots];
     zero(acc);
-    for (int k = k0 + threadIdx.x; k < k1; k += kTB) {
-      const int e = gl(a.inc)[k];
-      const V4<TM> j = ld4(a.J, e >> 1);
-      const double sg = (e & 1) ? -1.0 : 1.0;
+    for (int k = threadIdx.x; k < k1 - k0; k += kTB) {
+      const bool bSide = k < nd;
+      const V4<TM> j = bSide ? ld4(a.JB, jb0 + k) : ld4(a.J, c0 + k - nd);
+      const double sg = bSide ? -1.0 : 1.0;
       acc[0] += sg * (double)j.x;
       acc[1] += sg * (double)j.y;
       acc[2] += sg * (double)j.z;
@@ -902,6 +985,7 @@ __global__ __launch_bounds__(kTC / R, kP1MinWaves<R>) void kP1(const Args<TM>* _
     // are never used
     double gk[R][3], gm[R][3];
     TM rv[R][3], nk[R][3], nm[R][3], fc[R][6];
+    int pb[R];  // the contacts' B-side slots, loaded with the rest
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       const long q = (long)tile.start + threadIdx.x + u * NT;
@@ -919,11 +1003,13 @@ __global__ __launch_bounds__(kTC / R, kP1MinWaves<R>) void kP1(const Args<TM>* _
       const long qf = (valid && !bil) ? q : 0;
 #pragma unroll
       for (int k = 0; k < 6; ++k) fc[u][k] = k < NF ? gl(a.frame)[k * a.cap + qf] : (TM)0;
+      pb[u] = gl(a.posB)[qf];
     }
     const double bp = 1.0 + S.beta, bm = -S.beta;
     const double t = S.t;
-    double acc[kSlots];
-    zero(acc);
+    double acc[R][kSlots];  // per row of the thread: see blockReduceR
+#pragma unroll
+    for (int u = 0; u < R; ++u) zero(acc[u]);
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       const long q = (long)tile.start + threadIdx.x + u * NT;
@@ -959,11 +1045,11 @@ __global__ __launch_bounds__(kTC / R, kP1MinWaves<R>) void kP1(const Args<TM>* _
           } else {
             e = fabs(w0);
           }
-          acc[7] = fmax(acc[7], e == e ? e : INFINITY);
+          acc[u][7] = fmax(acc[u][7], e == e ? e : INFINITY);
         } else {
           const double w1 = (double)nk[u][1] + (double)rv[u][1];
           const double w2 = (double)nk[u][2] + (double)rv[u][2];
-          acc[7] = fmax(acc[7], residualContact(gk[u][0], gk[u][1], gk[u][2], w0, w1, w2, D.mu, D.gdiff, D.igdiff));
+          acc[u][7] = fmax(acc[u][7], residualContact(gk[u][0], gk[u][1], gk[u][2], w0, w1, w2, D.mu, D.gdiff, D.igdiff));
         }
       }
       if (!bil) projectCone<double>(gn[0], gn[1], gn[2], D.mu);
@@ -975,24 +1061,24 @@ __global__ __launch_bounds__(kTC / R, kP1MinWaves<R>) void kP1(const Args<TM>* _
           const double dy = gn[rr] - y[rr];
           if (kMixed) {
             gl(a.delta)[f] = (TM)dy;
-            acc[1] += dy * ny[rr];
+            acc[u][1] += dy * ny[rr];
           } else {
-            acc[0] += gn[rr] * (double)rv[u][rr];
-            acc[1] += y[rr] * ny[rr];
-            acc[2] += y[rr] * (double)rv[u][rr];
-            acc[3] += gg[rr] * dy;
+            acc[u][0] += gn[rr] * (double)rv[u][rr];
+            acc[u][1] += y[rr] * ny[rr];
+            acc[u][2] += y[rr] * (double)rv[u][rr];
+            acc[u][3] += gg[rr] * dy;
           }
-          acc[4] += dy * dy;
-          acc[5] += gg[rr] * (gn[rr] - gk[u][rr]);
+          acc[u][4] += dy * dy;
+          acc[u][5] += gg[rr] * (gn[rr] - gk[u][rr]);
         }
       }
       if (!bil) {
         TM F[9];
         frameFrom(fc[u], D.geomMode, F);
-        storeJ(a, (int)q, F, (TM)gn[0], (TM)gn[1], (TM)gn[2]);
+        storeJ(a, (int)q, pb[u], F, (TM)gn[0], (TM)gn[1], (TM)gn[2]);
       }
     }
-    writePartials<NT>(acc, TILE_PART(a.part1[par], tile));
+    writePartialsR<NT, R>(acc, TILE_PART(a.part1[par], tile));
   };
   if (T.single) {
     // one slot: the grid has a workgroup for every tile the slot's capacity
@@ -1062,10 +1148,13 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
       tA[u] = ld4(a.tmp, ca[u]);
       tB[u] = ld4(a.tmp, cb[u]);
     }
-    double acc[kSlots];
-    zero(acc);
+    // only slot 6 is used: one number per tile, packed so that the
+    // decision's loads of all tiles' partials touch few cache lines; one
+    // accumulator per row of the thread (see blockReduceR)
+    double a6[R][1];
 #pragma unroll
     for (int u = 0; u < R; ++u) {
+      a6[u][0] = 0.0;
       const long q = (long)tile.start + threadIdx.x + u * NT;
       const bool own = q < D.nc;
       TM F[9];
@@ -1078,32 +1167,30 @@ __global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, T
                                : g[u][0] * (double)n0 + g[u][1] * (double)n1 + g[u][2] * (double)n2;
       if (own) {
         Nn[ROW(a, 0, q)] = n0; Nn[ROW(a, 1, q)] = n1; Nn[ROW(a, 2, q)] = n2;
-        acc[6] += c6;
+        a6[u][0] += c6;
       } else if (q < items) {
         const long f = BROW(a, q - D.nc);
         const TM nv = bilRow(a, (int)(q - D.nc));
         Nn[f] = nv;
-        acc[6] += (kMixed ? (double)gl(a.delta)[f] : Gn[f]) * (double)nv;
+        a6[u][0] += (kMixed ? (double)gl(a.delta)[f] : Gn[f]) * (double)nv;
       }
     }
-    // only slot 6 is used: one number per tile, packed so that the
-    // decision's loads of all tiles' partials touch few cache lines
-    double a6[1] = {acc[6]};
     __shared__ double t6[1];
-    blockReduce<NT, 1>(a6, t6);
+    blockReduceR<NT, R, 1>(a6, t6);
     if (threadIdx.x == 0) gl(a.q3[par])[tile.start / kTC] = t6[0];
   };
   if (T.single) {
-    // one slot: every workgroup takes its tile (the grid covers the slot's
-    // tile capacity; tiles past the row count store only their partial,
-    // which no decision reads) with no test of the view's done flag, so
-    // that no branch waits on the view or the tile count before the loads.
-    // After the solve has ended the pass's stores are simply not used: N of
-    // ring slot nxt is never gamma-hat's, and the final kernels rebuild what
-    // they read.
+    // one slot: the grid covers the slot's tile capacity, and the tiles go
+    // to the workgroups in XCD-contiguous order (the tile count arrives
+    // with the table fields the loads need anyway); workgroups past the
+    // count have no tile.  There is no test of the view's done flag, so that
+    // no branch waits on the view before the loads.  After the solve has
+    // ended the pass's stores are simply not used: N of ring slot nxt is
+    // never gamma-hat's, and the final kernels rebuild what they read.
     const PassView S = loadView(T, 0, par ^ 1);
     const Args<TM>& a = A[0];
-    tileBody(Tile{0, (int)blockIdx.x * kTC}, S, a);
+    const TileRange tr = tilesOf(T, a.nTiles, gridDim.x);
+    for (int ti = tr.first; ti < tr.end; ti += tr.step) tileBody(Tile{0, ti * kTC}, S, a);
     return;
   }
   TILE_LOOP(T) {
@@ -1357,6 +1444,164 @@ void enqueuePasses(const Plan<TM>& P, hipStream_t st, int n, hipEvent_t* ev, int
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// The schedule tuner.  No schedule changes a result (see Schedule), so
+// candidate schedules are tried on successive chunks of live solves, each
+// chunk timed with events, and the solve goes on as it would under any of
+// them.  Once every candidate of a class has a measurement, the fastest per
+// pass and per active row becomes that class's schedule, kept for the
+// process and, per device, appended to a text file ($DVIDE_SCHEDULE_FILE,
+// else $HOME/.cache/dvide/schedules-<arch>-<CUs>.txt), from which later
+// processes take it without measuring.  A class is the device, the build,
+// one slot or several, and log2 of the rows solved.  DVIDE_TUNE=0 keeps the
+// first candidate of each list.
+
+namespace {
+
+const Schedule kOneSlot[] = {{1, 1, 0, 4}, {1, 0, 0, 4}, {2, 1, 0, 4}, {2, 0, 0, 4}};
+const Schedule kManySlots[] = {{1, 1, 32, 32}, {1, 0, 32, 32}, {2, 1, 32, 32},
+                               {1, 1, 16, 16}, {1, 1, 64, 32}, {2, 1, 64, 32}};
+constexpr int kOneSlotN = (int)(sizeof(kOneSlot) / sizeof(kOneSlot[0]));
+constexpr int kManySlotsN = (int)(sizeof(kManySlots) / sizeof(kManySlots[0]));
+
+struct TuneClass {
+  std::vector<double> score;  // per candidate: us per pass per million active rows (< 0: not measured)
+  int best = -1;              // the chosen candidate (-1: still measuring)
+  Schedule sched;             // its schedule (also when read from the file)
+};
+
+struct Tuner {
+  std::mutex m;
+  std::map<std::string, TuneClass> classes;
+  std::map<int, std::string> tags;  // device -> "<arch>-<CUs>"
+  std::map<std::string, bool> loaded;
+};
+
+Tuner& tuner() {
+  static Tuner t;
+  return t;
+}
+
+bool tuningEnabled() {
+  const char* e = getenv("DVIDE_TUNE");
+  return !(e && e[0] == '0');
+}
+
+std::string scheduleFile(const std::string& tag) {
+  const char* f = getenv("DVIDE_SCHEDULE_FILE");
+  if (f && f[0]) return f;
+  const char* home = getenv("HOME");
+  if (!home || !home[0]) return "";
+  return std::string(home) + "/.cache/dvide/schedules-" + tag + ".txt";
+}
+
+// caller holds the tuner's lock
+std::string deviceTag(Tuner& T) {
+  int dev = 0;
+  DV_HIP(hipGetDevice(&dev));
+  auto it = T.tags.find(dev);
+  if (it != T.tags.end()) return it->second;
+  hipDeviceProp_t p;
+  DV_HIP(hipGetDeviceProperties(&p, dev));
+  std::string arch = p.gcnArchName;
+  arch = arch.substr(0, arch.find(':'));
+  const std::string tag = arch + "-" + std::to_string(p.multiProcessorCount);
+  T.tags[dev] = tag;
+  if (!T.loaded[tag]) {
+    T.loaded[tag] = true;
+    const std::string path = scheduleFile(tag);
+    if (FILE* f = path.empty() ? nullptr : fopen(path.c_str(), "r")) {
+      char key[256];
+      Schedule sc;
+      double score;
+      while (fscanf(f, "%255s %d %d %d %d %lf", key, &sc.rpt, &sc.xcd, &sc.wgc, &sc.wgb, &score) == 6) {
+        if (sc.rpt < 1 || sc.rpt > 2 || sc.wgc < 0 || sc.wgb < 1) continue;
+        TuneClass& c = T.classes[key];
+        c.best = 0;  // decided
+        c.sched = sc;
+      }
+      fclose(f);
+    }
+  }
+  return tag;
+}
+
+void persist(const std::string& tag, const std::string& key, const Schedule& sc, double score) {
+  const std::string path = scheduleFile(tag);
+  if (path.empty()) return;
+  if (!getenv("DVIDE_SCHEDULE_FILE")) {  // create $HOME/.cache/dvide
+    const std::string dir = path.substr(0, path.rfind('/'));
+    (void)mkdir(dir.substr(0, dir.rfind('/')).c_str(), 0755);
+    (void)mkdir(dir.c_str(), 0755);
+  }
+  if (FILE* f = fopen(path.c_str(), "a")) {
+    fprintf(f, "%s %d %d %d %d %.6g\n", key.c_str(), sc.rpt, sc.xcd, sc.wgc, sc.wgb, score);
+    fclose(f);
+  }
+}
+
+// The schedule of a solve: fixed rows per thread, the class's decided
+// schedule, or (returned class, still measuring) the first candidate.
+TuneClass* chooseSchedule(Engine& E, long rows, bool allowTuning) {
+  const bool single = E.nSlots == 1;
+  const Schedule* cands = single ? kOneSlot : kManySlots;
+  const int n = single ? kOneSlotN : kManySlotsN;
+  E.sched = cands[0];
+  if (E.forcedRpt) {
+    E.sched.rpt = E.forcedRpt;
+    return nullptr;
+  }
+  Tuner& T = tuner();
+  std::lock_guard<std::mutex> lock(T.m);
+  int bucket = 0;
+  while ((2L << bucket) <= rows) ++bucket;
+  const std::string key = deviceTag(T) + ":" + (E.mixed ? "mixed" : "fp64") + ":" + (single ? "one" : "many") +
+                          ":" + std::to_string(bucket);
+  TuneClass& c = T.classes[key];
+  if (c.best >= 0) {
+    E.sched = c.sched;
+    return nullptr;
+  }
+  if (!allowTuning || !tuningEnabled()) return nullptr;
+  if ((int)c.score.size() != n) c.score.assign(n, -1.0);
+  E.tuneKey = key;
+  return &c;
+}
+
+// the next candidate to measure (not the one in flight), or -1
+int nextCandidate(TuneClass& c, int inFlight) {
+  std::lock_guard<std::mutex> lock(tuner().m);
+  for (int i = 0; i < (int)c.score.size(); ++i)
+    if (c.score[i] < 0 && i != inFlight) return i;
+  return -1;
+}
+
+// records a measurement; true once the class is decided (E.sched is then its schedule)
+bool recordCandidate(Engine& E, TuneClass& c, int i, double score) {
+  Tuner& T = tuner();
+  std::lock_guard<std::mutex> lock(T.m);
+  if (c.best >= 0) {
+    E.sched = c.sched;
+    return true;
+  }
+  if (c.score[i] < 0 || score < c.score[i]) c.score[i] = score;
+  int best = 0;
+  for (int k = 0; k < (int)c.score.size(); ++k) {
+    if (c.score[k] < 0) return false;
+    if (c.score[k] < c.score[best]) best = k;
+  }
+  const bool single = E.nSlots == 1;
+  c.best = best;
+  c.sched = (single ? kOneSlot : kManySlots)[best];
+  E.sched = c.sched;
+  persist(E.tuneKey.substr(0, E.tuneKey.find(':')), E.tuneKey, c.sched, c.score[best]);
+  return true;
+}
+
+const Schedule& candidate(const Engine& E, int i) { return (E.nSlots == 1 ? kOneSlot : kManySlots)[i]; }
+
+}  // namespace
+
 void GraphCache::reset() {
   if (exec) {
     if (launchedOn) (void)hipStreamSynchronize(launchedOn);
@@ -1408,6 +1653,11 @@ void Engine::release() {
     if (e) (void)hipEventDestroy(e);
     e = nullptr;
   }
+  for (auto& pair : tev)
+    for (hipEvent_t& e : pair) {
+      if (e) (void)hipEventDestroy(e);
+      e = nullptr;
+    }
   outHostCap = 0;
   graph.reset();
   burst.reset();
@@ -1428,9 +1678,10 @@ void prepareScene(Scene& s, Args<TM>& a) {
   const long tilesCap = std::max<long>(1, (itemsCap + kTC - 1) / kTC);
   s.partials.reserve((size_t)4 * kSlots * tilesCap + 2 * kSlots + 2 * tilesCap);
   s.work.reserve((size_t)5 * RsCap + 16);  // ring[4], gHat: FP64 in both solvers
-  // TM block: rn[4], r, delta, then J (per contact) and tmp (per body) of 4
-  // components 32-byte aligned, then (mixed) M^-1
-  const size_t tmElems = (size_t)6 * RsCap + 8 + 4 * ((size_t)std::max(s.ncCap, 1) + (size_t)std::max(s.nb, 1)) +
+  // TM block: rn[4], r, delta, then J and JB (per contact) and tmp (per
+  // body) of 4 components 32-byte aligned, then (mixed) M^-1
+  const size_t tmElems = (size_t)6 * RsCap + 8 +
+                         4 * (2 * (size_t)std::max(s.ncCap, 1) + (size_t)std::max(s.nb, 1)) +
                          (kMixed ? (size_t)s.nb + 1 : 0) + 16;
   TM* WM;
   if (kMixed) {
@@ -1452,7 +1703,7 @@ void prepareScene(Scene& s, Args<TM>& a) {
   a.active = 1;
   a.frame64 = s.frame.p;
   a.cap = s.ncCap;
-  a.cA = s.cA.p; a.cB = s.cB.p; a.iStart = s.iStart.p; a.inc = s.inc.p;
+  a.cA = s.cA.p; a.cB = s.cB.p; a.iStart = s.iStart.p; a.cStart = s.cStart.p; a.posB = s.posB.p;
   a.bilIncStart = s.bilIncStart.p; a.bilInc = s.bilInc.p; a.bilA = s.bilA.p; a.bilB = s.bilB.p;
   a.heavy = s.heavy.p; a.counters = s.counters.p;
   a.minv64 = s.minv.p;
@@ -1468,7 +1719,8 @@ void prepareScene(Scene& s, Args<TM>& a) {
   TM* after = WM + 6 * RsCap;
   uintptr_t p = (reinterpret_cast<uintptr_t>(after) + 31) & ~uintptr_t(31);
   a.J = reinterpret_cast<V4<TM>*>(p);
-  a.tmp = a.J + std::max<long>(s.ncCap, 1);
+  a.JB = a.J + std::max<long>(s.ncCap, 1);
+  a.tmp = a.JB + std::max<long>(s.ncCap, 1);
   if (kMixed) {
     a.frame = reinterpret_cast<TM*>(s.frame32.p);
     a.minv = reinterpret_cast<TM*>(a.tmp + std::max(s.nb, 1));
@@ -1529,6 +1781,9 @@ void buildSet(Engine& E, TileSet& set, hipStream_t st) {
 template <typename TM>
 Tables tablesOf(Engine& E, TileSet& set) {
   Tables T;
+  // the padding too: a Plan holding these tables is compared bytewise with
+  // the one a cached graph was captured with
+  std::memset(&T, 0, sizeof(T));
   T.items = set.items.p;
   T.bodies = set.bodies.p;
   T.heavy = set.heavy.p;
@@ -1538,6 +1793,7 @@ Tables tablesOf(Engine& E, TileSet& set) {
   T.st = reinterpret_cast<ApgdState*>(E.states.p);
   T.pv = reinterpret_cast<PassView*>(E.views.p);
   T.single = E.nSlots == 1 ? 1 : 0;
+  T.xcd = E.sched.xcd;
   return T;
 }
 
@@ -1558,12 +1814,12 @@ void grids(Engine& E, const TileSet& set, int& gc, int& gb) {
     gc = (int)tilesCap;
     gb = (int)std::max<long>(1, std::min<long>((a.dims.nb + kTB - 1) / kTB, 4L * E.numCU));
   } else {
-    // several slots: 32 workgroups per CU over the tile tables, each taking
-    // a few tiles, so that the table and view reads that start every tile
-    // overlap across many workgroups (512 scenes: the solve 330 ms per batch
-    // step instead of 386 ms with 2 and 4 per CU; 64 per CU is slower again)
-    gc = 32 * E.numCU;
-    gb = 32 * E.numCU;
+    // several slots: wgc and wgb workgroups per CU over the tile tables,
+    // each taking a few tiles, so that the table and view reads that start
+    // every tile overlap across many workgroups (512 scenes: the solve
+    // 330 ms per batch step with 32 per CU instead of 386 ms with 2 and 4)
+    gc = E.sched.wgc * E.numCU;
+    gb = E.sched.wgb * E.numCU;
   }
   (void)set;
 }
@@ -1576,7 +1832,7 @@ Plan<TM> activePlan(Engine& E) {
   P.T = tablesOf<TM>(E, E.active);
   grids<TM>(E, E.active, P.gc, P.gb);
   P.nSlots = E.nSlots;
-  P.rpt = E.rowsPerThread;
+  P.rpt = E.sched.rpt;
   return P;
 }
 
@@ -1614,21 +1870,26 @@ void startAdmitted(Engine& E, hipStream_t st) {
   rebuildActive<TM>(E, st);
 }
 
+// the captured graph of a chunk of passes under the current plan
+template <typename TM>
+void prepareChunk(Engine& E, hipStream_t st) {
+  const Plan<TM> P = activePlan<TM>(E);
+  if (E.noGraph || E.graph.matches(&P, sizeof(P))) return;
+  E.graph.reset();
+  hipGraph_t graph;
+  DV_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  enqueuePasses<TM>(P, st, kPassesPerChunk, nullptr, 0, nullptr);
+  DV_HIP(hipStreamEndCapture(st, &graph));
+  E.graph.instantiate(graph, "hipGraphInstantiate(apgd passes)");
+  E.graph.setKey(&P, sizeof(P));
+}
+
 template <typename TM>
 void runChunk(Engine& E, hipStream_t st, hipEvent_t* ev, int prof) {
-  const Plan<TM> P = activePlan<TM>(E);
   if (E.noGraph || ev) {
-    enqueuePasses<TM>(P, st, kPassesPerChunk, ev, prof, &E.burst);
+    enqueuePasses<TM>(activePlan<TM>(E), st, kPassesPerChunk, ev, prof, &E.burst);
   } else {
-    if (!E.graph.matches(&P, sizeof(P))) {
-      E.graph.reset();
-      hipGraph_t graph;
-      DV_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      enqueuePasses<TM>(P, st, kPassesPerChunk, nullptr, 0, nullptr);
-      DV_HIP(hipStreamEndCapture(st, &graph));
-      E.graph.instantiate(graph, "hipGraphInstantiate(apgd passes)");
-      E.graph.setKey(&P, sizeof(P));
-    }
+    prepareChunk<TM>(E, st);
     E.graph.launch(st);
   }
   E.launched += kPassesPerChunk;
@@ -1660,6 +1921,11 @@ void retireSlots(Engine& E, const std::vector<int>& slots, hipStream_t st) {
     s.stats.residual = S.best;
     s.stats.objective = S.objective;
     s.stats.passes = S.passes;
+    s.stats.rows_per_thread = E.sched.rpt;
+    s.stats.xcd_order = E.sched.xcd;
+    s.stats.wg_per_cu_contacts = E.nSlots > 1 ? E.sched.wgc : 0;
+    s.stats.wg_per_cu_bodies = E.sched.wgb;
+    s.stats.schedule_tuning = E.tuning;
     if (s.verbose) printf("  Iterations: %d Residual: %g\n", S.it, S.best);
     slotArgs<TM>(E, slot).active = 0;
     E.slotScene[slot] = nullptr;
@@ -1677,9 +1943,12 @@ void openT(Engine& E, int nSlots) {
 
 }  // namespace
 
-void engineOpen(Engine& E, int nSlots, bool mixed, int numCU, bool noGraph, int rowsPerThread) {
+void engineOpen(Engine& E, int nSlots, bool mixed, int numCU, bool noGraph, int forcedRpt) {
   if (nSlots < 1) throw DvError{DV_ERR_ARG, "an engine needs at least one slot"};
-  E.rowsPerThread = rowsPerThread == 2 ? 2 : 1;
+  E.forcedRpt = (forcedRpt == 1 || forcedRpt == 2) ? forcedRpt : 0;
+  E.sched = nSlots == 1 ? kOneSlot[0] : kManySlots[0];
+  if (E.forcedRpt) E.sched.rpt = E.forcedRpt;
+  E.tuning = 0;
   if (E.mixed != (mixed ? 1 : 0) || E.nSlots != nSlots) E.graph.reset();
   E.mixed = mixed ? 1 : 0;
   E.nSlots = nSlots;
@@ -1752,6 +2021,18 @@ void solveAll(Engine& E, const std::vector<Scene*>& scenes, hipStream_t st, int 
               std::vector<hipEvent_t>* profEv, dv_stats* profStats) {
   const int n = (int)scenes.size();
   for (int i = 0; i < n; ++i) engineAdmit(E, i, *scenes[i]);
+  // the schedule: fixed, decided for the solve's class, or measured on this
+  // solve's chunks (not while a kernel is being profiled)
+  long rows = 0;
+  std::vector<long> slotRows(n);
+  for (int i = 0; i < n; ++i) rows += slotRows[i] = (long)scenes[i]->nc + scenes[i]->nbil;
+  TuneClass* tc = chooseSchedule(E, rows, prof == 0);
+  E.tuning = tc ? 1 : 0;
+  if (tc)
+    for (auto& pair : E.tev)
+      for (hipEvent_t& e : pair)
+        if (!e) DV_HIP(hipEventCreate(&e));
+  E.chunkCand[0] = E.chunkCand[1] = -1;
   startAdmitted<TM>(E, st);
   hipEvent_t* ev = nullptr;
   if (prof && profEv) {
@@ -1770,14 +2051,28 @@ void solveAll(Engine& E, const std::vector<Scene*>& scenes, hipStream_t st, int 
   ApgdState* buf[2] = {outH, outH + E.nSlots};
   for (int b = 0; b < 2; ++b)
     if (!E.polled[b]) DV_HIP(hipEventCreateWithFlags(&E.polled[b], hipEventDisableTiming));
-  // a chunk, then the copy of the states after it into buf[b]
+  // a chunk, then the copy of the states after it into buf[b]; while the
+  // class is being measured, the chunk runs the next unmeasured candidate
+  // schedule between two timing events
   auto launch = [&](int b, bool timed) {
+    const int ci = tc ? nextCandidate(*tc, E.chunkCand[b ^ 1]) : -1;
+    if (ci >= 0) {
+      E.sched = candidate(E, ci);
+      if (!timed) prepareChunk<TM>(E, st);  // capture outside the timed interval
+      DV_HIP(hipEventRecord(E.tev[b][0], st));
+    }
+    E.chunkCand[b] = ci;
     runChunk<TM>(E, st, timed ? ev : nullptr, prof);
+    if (ci >= 0) DV_HIP(hipEventRecord(E.tev[b][1], st));
     DV_HIP(hipMemcpyAsync(buf[b], E.out.p, sizeof(ApgdState) * E.nSlots, hipMemcpyDeviceToHost, st));
     DV_HIP(hipEventRecord(E.polled[b], st));
   };
   bool sampled = false, curTimed = false;
   int cur = 0;
+  // a measured chunk counts when no slot finished during it: then every
+  // active slot ran all of its passes
+  int doneBefore = 0;
+  double activeRows = (double)rows;
   launch(cur, false);
   for (;;) {
     // The next chunk is queued before this one's states are read, so the GPU
@@ -1795,9 +2090,21 @@ void solveAll(Engine& E, const std::vector<Scene*>& scenes, hipStream_t st, int 
       profStats->prof_kernel_count += prof == 4 ? 1 : kProfBurst;
     }
     passesBefore = last[0].passes;
-    bool all = true;
-    for (int i = 0; i < n && all; ++i) all = last[i].done != 0;
-    if (all) break;
+    int doneNow = 0;
+    double rowsLeft = 0;
+    for (int i = 0; i < n; ++i) {
+      if (last[i].done) ++doneNow;
+      else rowsLeft += (double)slotRows[i];
+    }
+    if (tc && E.chunkCand[cur] >= 0 && doneNow == doneBefore && doneNow < n) {
+      float ms = 0;
+      DV_HIP(hipEventElapsedTime(&ms, E.tev[cur][0], E.tev[cur][1]));
+      const double score = 1e3 * ms / kPassesPerChunk / activeRows * 1e6;
+      if (recordCandidate(E, *tc, E.chunkCand[cur], score)) tc = nullptr;
+    }
+    doneBefore = doneNow;
+    activeRows = rowsLeft;
+    if (doneNow == n) break;
     if (E.launched > passCap) throw DvError{DV_ERR_DIVERGED, "APGD exceeded its pass budget"};
     cur ^= 1;
     curTimed = false;
@@ -1821,6 +2128,49 @@ void solveFusedMany(Engine& E, const std::vector<Scene*>& scenes, bool mixed, hi
   engineOpen(E, (int)scenes.size(), mixed, numCU, noGraph, scenes[0]->rowsPerThread);
   if (mixed) solveAll<float>(E, scenes, st, 0, nullptr, nullptr);
   else solveAll<double>(E, scenes, st, 0, nullptr, nullptr);
+}
+
+namespace {
+
+// N x by the kernels a solver pass runs, on the assembled scene alone: the
+// setup takes gamma_0 = x (warm start from the staging copy of x) and stores
+// J = F^T x with the store P1 uses; then P2 (kBody<TM, 2>) forms tmp = M^-1
+// (sum_A J - sum_B J + bilateral rows of x) and P3 (kP3) the rows
+// (N x)_c = F_c (tmp_A - tmp_B) and the bilateral rows, into N of ring slot 1
+// (the pass's gamma_new slot, which gets a copy of x).  y receives them in
+// the reference's flat layout, as TM values.
+template <typename TM>
+void passProductT(Scene& s, const double* xDev, double* yHost) {
+  Engine& E = s.engine;
+  const hipStream_t st = s.stream;
+  engineOpen(E, 1, sizeof(TM) < sizeof(double), s.numCU, true, s.rowsPerThread == 2 ? 2 : 1);
+  engineAdmit(E, 0, s);
+  Args<TM>& a = slotArgs<TM>(E, 0);
+  a.gammaIO = const_cast<double*>(xDev);  // read only: warm start from x
+  a.dims.warm = 1;
+  a.dims.maxIt = 1;  // a live state, so that P2 gathers
+  startAdmitted<TM>(E, st);
+  DV_HIP(hipMemcpyAsync(a.ring[1], a.ring[0], sizeof(double) * a.ringStride, hipMemcpyDeviceToDevice, st));
+  const Plan<TM> P = activePlan<TM>(E);
+  launchPass<TM>(P, st, 2, 0, false);  // kBody<TM, 2> on the view the setup left (nxt = 1)
+  launchP3<TM>(P, st, 0);
+  std::vector<TM> rn((size_t)a.ringStride);
+  DV_HIP(hipMemcpyAsync(rn.data(), a.rn[1], sizeof(TM) * rn.size(), hipMemcpyDeviceToHost, st));
+  DV_HIP(hipStreamSynchronize(st));
+  DV_HIP(hipGetLastError());
+  for (int c = 0; c < s.nc; ++c)
+    for (int rr = 0; rr < 3; ++rr) yHost[s.nbil + 3L * c + rr] = (double)rn[(size_t)rr * a.cap + c];
+  for (int q = 0; q < s.nbil; ++q) yHost[q] = (double)rn[3 * (size_t)a.cap + q];
+  a.active = 0;
+  E.slotScene[0] = nullptr;
+}
+
+}  // namespace
+
+void passProduct(Scene& s, bool mixed, const double* xDev, double* yHost) {
+  if ((long)s.nc + s.nbil == 0) return;
+  if (mixed) passProductT<float>(s, xDev, yHost);
+  else passProductT<double>(s, xDev, yHost);
 }
 
 void solveFused(Scene& s, bool mixed) {

@@ -262,8 +262,8 @@ int dv_scene_destroy(dv_scene* h) {
   for (auto* b : dbufs) b->release();
   dv::DevBuf<int>* ibufs[] = {&s.family, &s.smallIds, &s.bigIds, &s.bilA, &s.bilB, &s.bilIncStart,
                               &s.bilInc, &s.cellItems, &s.cntUp, &s.cntDown, &s.cStart, &s.iStart,
-                              &s.scanTmp, &s.counters, &s.hostTotals, &s.cA, &s.cB, &s.inc, &s.heavy,
-                              &s.bigChunks};
+                              &s.scanTmp, &s.counters, &s.hostTotals, &s.cA, &s.cB, &s.inc, &s.posB,
+                              &s.heavy, &s.bigChunks};
   for (auto* b : ibufs) b->release();
   s.slotOf.release(); s.slotCount.release(); s.slotStart.release(); s.slotFill.release();
   s.frame32.release(); s.work32.release();
@@ -671,6 +671,22 @@ int dv_schur_product(dv_scene* h, const double* x, double* y, size_t n) {
   });
 }
 
+int dv_pass_product(dv_scene* h, int build, const double* x, double* y, size_t n) {
+  return dv::guarded(h, [&](Scene& s) {
+    if (build != DV_APGD_FP64 && build != DV_APGD_MIXED)
+      return dv::fail(s, DV_ERR_ARG, "the pass kernels belong to the FP64 and mixed builds");
+    if (!s.initialized) return dv::fail(s, DV_ERR_STATE, "not initialized");
+    if (n != (size_t)s.R || !x || !y) return dv::fail(s, DV_ERR_ARG, "x and y need nbil + 3*nc entries");
+    if (n == 0) return DV_OK;
+    dv::DevBuf<double> xs;
+    xs.reserve(n);
+    DV_HIP(hipMemcpyAsync(xs.p, x, sizeof(double) * n, hipMemcpyHostToDevice, s.stream));
+    dv::passProduct(s, build == DV_APGD_MIXED, xs.p, y);
+    xs.release();
+    return DV_OK;
+  });
+}
+
 int dv_project_cone(dv_scene* h, double* g, size_t n, double mu) {
   return dv::guarded(h, [&](Scene& s) {
     if (!g) return DV_ERR_ARG;
@@ -771,11 +787,11 @@ void needScene(dv_scene* h) {
 
 // Device memory the solver of a slot needs beyond the scene's own state:
 // the FP64 iterate ring (4 slots + gammaHat), the product vectors (4 N gamma,
-// r, delta), J, tmp, the FP32 frames and the partial sums.
+// r, delta), J and JB, tmp, the FP32 frames and the partial sums.
 double solverBytes(const Scene& s, bool mixed) {
   const double rows = 3.0 * std::max(s.ncCap, 1) + s.nbil;
   const double tm = mixed ? 4 : 8;
-  return rows * (5 * 8 + 6 * tm) + std::max(s.ncCap, 1) * (4 * tm + (mixed ? 24 : 0)) +
+  return rows * (5 * 8 + 6 * tm) + std::max(s.ncCap, 1) * (8 * tm + (mixed ? 24 : 0)) +
          std::max(s.nb, 1) * (5 * tm) + 4.0 * 8 * 8 * (1 + (rows / 3 + 1023) / 1024);
 }
 
@@ -854,6 +870,11 @@ void batchStep(dv_batch& B) {
     B.stats.max_iterations = std::max(B.stats.max_iterations, s.stats.iterations);
   }
   B.stats.num_scenes = (int)B.scenes.size();
+  B.stats.rows_per_thread = B.engine.sched.rpt;
+  B.stats.xcd_order = B.engine.sched.xcd;
+  B.stats.wg_per_cu_contacts = B.engine.sched.wgc;
+  B.stats.wg_per_cu_bodies = B.engine.sched.wgb;
+  B.stats.schedule_tuning = B.engine.tuning;
   B.stats.ms_assembly = msAsm;
   B.stats.ms_solve = msSolve;
   B.stats.ms_update = msSince(t2);

@@ -295,14 +295,19 @@ __global__ __launch_bounds__(kT) void emitBigChunks(CdArgs a, EmitOut o, int nch
   }
 }
 
-// incidences of body g in ascending contact order: (c << 1) | side, side 1 = B
+// incidences of body g in ascending contact order: (c << 1) | side, side 1 =
+// B (first the contacts of its lower-numbered partners' rows, then its own
+// row); and each contact's slot in the B-side order, posB[c] = (iStart[g] -
+// cStart[g]) + q for the q-th B incidence of g, so that the B incidences of
+// every body are contiguous there
 __global__ __launch_bounds__(kT) void buildIncidence(int nb, const int* cStart, const int* iStart,
                                                      const int* cntUp, const int* cntDown,
-                                                     const int* cB, int* inc, int* heavy,
+                                                     const int* cB, int* inc, int* posB, int* heavy,
                                                      int* counters) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nb) return;
   const int i0 = iStart[g], nd = cntDown[g], nu = cntUp[g];
+  const int b0 = i0 - cStart[g];
   for (int q = 0; q < nd; ++q) {
     int j = inc[i0 + q];
     int lo = cStart[j], hi = cStart[j] + cntUp[j];
@@ -311,6 +316,7 @@ __global__ __launch_bounds__(kT) void buildIncidence(int nb, const int* cStart, 
       if (cB[mid] < g) lo = mid + 1; else hi = mid;
     }
     inc[i0 + q] = (lo << 1) | 1;
+    posB[lo] = b0 + q;
   }
   const int c0 = cStart[g];
   for (int q = 0; q < nu; ++q) inc[i0 + nd + q] = (c0 + q) << 1;
@@ -447,12 +453,13 @@ void collideEmit(Scene& s) {
   if (nc > s.ncCap || s.ncCap == 0) {
     int cap = std::max(1024, nc + nc / 4);
     s.cA.release(); s.cB.release(); s.frame.release(); s.gap.release(); s.loc.release();
-    s.frame32.release(); s.inc.release();
+    s.frame32.release(); s.inc.release(); s.posB.release();
     s.cA.reserve(cap); s.cB.reserve(cap);
     s.frame.reserve((size_t)9 * cap);
     s.gap.reserve(cap);
     s.loc.reserve((size_t)3 * cap);
     s.inc.reserve((size_t)2 * cap);
+    s.posB.reserve(cap);
     s.ncCap = cap;
   }
   EmitOut o;
@@ -464,7 +471,7 @@ void collideEmit(Scene& s) {
   if (s.nbig)
     hipLaunchKernelGGL(emitBigChunks, dim3(nchunks, s.nbig), dim3(kT), 0, st, a, o, nchunks, chUp, chDown);
   hipLaunchKernelGGL(buildIncidence, dim3(blocksFor(s.nb, kT)), dim3(kT), 0, st, s.nb, s.cStart.p,
-                     s.iStart.p, s.cntUp.p, s.cntDown.p, s.cB.p, s.inc.p, s.heavy.p, s.counters.p);
+                     s.iStart.p, s.cntUp.p, s.cntDown.p, s.cB.p, s.inc.p, s.posB.p, s.heavy.p, s.counters.p);
   DV_HIP(hipGetLastError());
 }
 

@@ -109,6 +109,17 @@ struct TileSet {
   void release();
 };
 
+// Launch schedule of the solver's pass kernels.  None of its choices changes
+// a result: rows per thread only moves rows between lanes (the partial sums
+// keep the one-row-per-thread order), and the tile order and grid sizes only
+// decide which workgroup takes which tile.
+struct Schedule {
+  int rpt = 1;  // contact rows per thread of P1 and P3 (1 or 2)
+  int xcd = 1;  // XCD-contiguous tile order for the gathering kernels
+  int wgc = 32; // several slots: contact-kernel workgroups per CU
+  int wgb = 32; // several slots: body-kernel workgroups per CU (one slot: 4)
+};
+
 // The graph-driven APGD solver over a table of scene slots
 // (dv_apgd_fused.hip): the device-resident table, the tile sets of the
 // active, newly admitted and retiring slots, the per-slot states the host
@@ -116,7 +127,12 @@ struct TileSet {
 // their setup) and leave (after their finalisation) between chunks.
 struct Engine {
   int mixed = -1;       // 1: FP32 products; 0: FP64
-  int rowsPerThread = 1;  // contact kernels: rows per thread (1 or 2)
+  Schedule sched;       // of the chunk being launched
+  int forcedRpt = 0;    // 1 or 2: a fixed schedule with these rows per thread; 0: tuned
+  int tuning = 0;       // the last solve measured candidate schedules
+  hipEvent_t tev[2][2] = {};  // timing of each in-flight chunk while tuning
+  int chunkCand[2] = {-1, -1};  // candidate schedule of each in-flight chunk (-1: none)
+  std::string tuneKey;  // the class being measured
   int nSlots = 0;
   std::vector<char> hostTable;  // Args<TM>[nSlots]
   std::vector<Scene*> slotScene;
@@ -195,6 +211,7 @@ struct Scene {
   DevBuf<double> loc;    // 3 SoA arrays of ncCap
   DevBuf<float> frame32; // FP32 copy for the mixed solver
   DevBuf<int> inc;       // per-body incidences (contact << 1 | side)
+  DevBuf<int> posB;      // per contact: its slot in the B-side order (see buildIncidence)
   DevBuf<int> heavy;     // bodies processed by a whole workgroup
   DevBuf<int> bigChunks; // per (big geometry, chunk) partner counts
 
@@ -236,6 +253,9 @@ void minvTimes(Scene& s, const double* x, double* y);  // y = M^-1 x per DOF
 // solvers (dv_apgd_plain.hip, dv_apgd_fused.hip)
 void solvePlain(Scene& s);
 void solveFused(Scene& s, bool mixed);
+// N x by the graph-driven solver's own pass kernels (J store, P2, P3) on the
+// assembled scene; x is a flat device array, y a flat host array
+void passProduct(Scene& s, bool mixed, const double* xDev, double* yHost);
 // one solve of several scenes in the same launches (on stream st); each
 // scene's results equal those of solveFused on it alone
 void solveFusedMany(Engine& E, const std::vector<Scene*>& scenes, bool mixed, hipStream_t st, int numCU,
@@ -243,7 +263,9 @@ void solveFusedMany(Engine& E, const std::vector<Scene*>& scenes, bool mixed, hi
 // continuous batching on an engine (dv_apgd_fused.hip): slots are admitted
 // (scene assembled, setup launched) and retired (gamma written back, stats
 // filled) between chunks of passes
-void engineOpen(Engine& E, int nSlots, bool mixed, int numCU, bool noGraph, int rowsPerThread);
+// forcedRpt: 1 or 2 fixes the schedule (with that many rows per thread);
+// 0 lets the schedule tuner choose
+void engineOpen(Engine& E, int nSlots, bool mixed, int numCU, bool noGraph, int forcedRpt);
 void engineAdmit(Engine& E, int slot, Scene& s);
 void engineStartAdmitted(Engine& E, hipStream_t st);
 void engineRunChunk(Engine& E, hipStream_t st);

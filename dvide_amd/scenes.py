@@ -135,3 +135,42 @@ def cloud(seed: int, n: int = 500, edge: float = 1.0, rmin=0.02, rmax=0.06) -> S
 
 def free_fall(seed: int, n: int = 10) -> Scene:
     return generate(FREE_FALL, seed, n)
+
+
+def layer(seed: int, per_side: int = 16, rmin: float = 0.03, rmax: float = 0.06, mu: float = 0.3,
+          tol: float = 1e-6, max_iterations: int = 100000) -> Scene:
+    """One layer of spheres resting on a fixed floor box, far enough apart
+    that each touches only the floor: every contact is the only one of its
+    sphere, so N = D M^-1 D^T is block diagonal with blocks I / m and the
+    contact impulses gamma are unique.  Half of the spheres slide (lateral
+    speeds U[0.05, 0.5] m/s, above mu g h), the others stick (below 1e-3 m/s).
+    Radii U[rmin, rmax], density 2500, y up, h = 1e-3; numpy's PCG64 seeded
+    with `seed`."""
+    rng = np.random.default_rng(seed)
+    n = per_side * per_side
+    pitch = 2.0 * rmax + 0.05
+    r = rmin + (rmax - rmin) * rng.random(n)
+    ix, iz = np.divmod(np.arange(n), per_side)
+    pos = np.zeros((n + 1, 3))
+    pos[1:, 0] = (ix - 0.5 * (per_side - 1)) * pitch
+    pos[1:, 1] = r  # touching the floor's top face at y = 0
+    pos[1:, 2] = (iz - 0.5 * (per_side - 1)) * pitch
+    half = 0.5 * per_side * pitch + 0.1
+    pos[0] = (0.0, -0.05, 0.0)
+    geom = np.zeros((n + 1, 3))
+    geom[0] = (half, 0.05, half)
+    geom[1:, 0] = r
+    ang = 2.0 * np.pi * rng.random(n)
+    speed = np.where(np.arange(n) % 2 == 0, 0.05 + 0.45 * rng.random(n), 1e-3 * rng.random(n))
+    vel = np.zeros((n + 1, 3))
+    vel[1:, 0] = speed * np.cos(ang)
+    vel[1:, 2] = speed * np.sin(ang)
+    mass = np.ones(n + 1)
+    mass[1:] = 2500.0 * 4.0 / 3.0 * np.pi * r ** 3
+    fixed = np.zeros(n + 1, np.int32)
+    fixed[0] = 1
+    sc = Scene(pos=pos, vel=vel, mass=mass, geom=geom, fixed=fixed, family=-np.ones(n + 1, np.int32),
+               bil_dofs=np.zeros((0, 2), np.int32), bil_info=np.zeros((0, 2)))
+    sc.params = dict(h=1e-3, mu=mu, tol=tol, max_iterations=max_iterations, gravity=(0.0, -9.81, 0.0),
+                     envelope=1e-3, bins=20, steps=1, warm_start=False)
+    return sc

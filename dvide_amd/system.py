@@ -33,8 +33,8 @@ EXPORTED = (
     "dv_scene_create", "dv_scene_destroy", "dv_last_error", "dv_add_bodies", "dv_add_bilateral_dof",
     "dv_set_bins_per_axis", "dv_set_param", "dv_get_param", "dv_initialize", "dv_step",
     "dv_num_bodies", "dv_get_bodies", "dv_num_contacts", "dv_get_stats", "dv_get_vector", "dv_set_vector",
-    "dv_get_contacts", "dv_ccp_violation", "dv_assemble", "dv_schur_product", "dv_project_cone",
-    "dv_solve", "dv_get_jacobian", "dv_contact_forces", "dv_energies",
+    "dv_get_contacts", "dv_ccp_violation", "dv_assemble", "dv_schur_product",
+    "dv_pass_product", "dv_project_cone", "dv_solve", "dv_get_jacobian", "dv_contact_forces", "dv_energies",
     "dv_batch_create", "dv_batch_destroy", "dv_batch_last_error", "dv_batch_add", "dv_batch_step",
     "dv_batch_get_stats",
     "dv_server_create", "dv_server_destroy", "dv_server_last_error", "dv_server_submit", "dv_server_poll",
@@ -52,6 +52,9 @@ class DvStats(ctypes.Structure):
         ("ms_update", ctypes.c_double), ("time", ctypes.c_double),
         ("prof_kernel_ms", ctypes.c_double), ("prof_kernel_count", ctypes.c_int),
         ("passes", ctypes.c_int),
+        ("rows_per_thread", ctypes.c_int), ("xcd_order", ctypes.c_int),
+        ("wg_per_cu_contacts", ctypes.c_int), ("wg_per_cu_bodies", ctypes.c_int),
+        ("schedule_tuning", ctypes.c_int),
     ]
 
 
@@ -71,6 +74,9 @@ class DvBatchStats(ctypes.Structure):
         ("total_contacts", ctypes.c_long), ("total_iterations", ctypes.c_long),
         ("ms_total", ctypes.c_double), ("ms_assembly", ctypes.c_double),
         ("ms_solve", ctypes.c_double), ("ms_update", ctypes.c_double),
+        ("rows_per_thread", ctypes.c_int), ("xcd_order", ctypes.c_int),
+        ("wg_per_cu_contacts", ctypes.c_int), ("wg_per_cu_bodies", ctypes.c_int),
+        ("schedule_tuning", ctypes.c_int),
     ]
 
 
@@ -116,6 +122,7 @@ def load_library(path: str = LIB_PATH):
         "dv_ccp_violation": ([vp, _dp], ctypes.c_int),
         "dv_assemble": ([vp, _ip], ctypes.c_int),
         "dv_schur_product": ([vp, _dp, _dp, ctypes.c_size_t], ctypes.c_int),
+        "dv_pass_product": ([vp, ctypes.c_int, _dp, _dp, ctypes.c_size_t], ctypes.c_int),
         "dv_project_cone": ([vp, _dp, ctypes.c_size_t, ctypes.c_double], ctypes.c_int),
         "dv_solve": ([vp], ctypes.c_int),
         "dv_get_jacobian": ([vp, _ip, _ip, _dp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _ip, _ip],
@@ -486,6 +493,14 @@ class System:
         x = _f64(x)
         y = np.zeros_like(x)
         self._check(self._L.dv_schur_product(self._h, x.ctypes.data_as(_dp), y.ctypes.data_as(_dp), x.size))
+        return y
+
+    def pass_product(self, x, build):
+        """N x by the graph-driven solver's pass kernels (BUILD_FP64 or BUILD_MIXED)."""
+        x = _f64(x)
+        y = np.zeros_like(x)
+        self._check(self._L.dv_pass_product(self._h, int(build), x.ctypes.data_as(_dp), y.ctypes.data_as(_dp),
+                                            x.size))
         return y
 
 

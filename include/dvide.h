@@ -117,6 +117,16 @@ typedef struct dv_stats {
   double prof_kernel_ms;  /* DV_PARAM_PROFILE: summed duration of the timed launches */
   int prof_kernel_count;  /* ... and their number */
   int passes;             /* solver passes (Schur products) of the step */
+  /* launch schedule of the graph-driven builds' last chunk of passes (none
+     changes a result): contact rows per thread, XCD-contiguous tile order,
+     and, for several slots, workgroups per CU of the contact and body
+     kernels; schedule_tuning is 1 when the solve measured candidate
+     schedules (see DVIDE_SCHEDULE_FILE in dvide_amd/csrc/dv_apgd_fused.hip) */
+  int rows_per_thread;
+  int xcd_order;
+  int wg_per_cu_contacts;
+  int wg_per_cu_bodies;
+  int schedule_tuning;
 } dv_stats;
 
 /* --- scene lifetime ------------------------------------------------------ */
@@ -211,6 +221,15 @@ int dv_assemble(dv_scene* s, int* nc);
  * row order, as a row-sorted COO SpMV sums them. */
 int dv_schur_product(dv_scene* s, const double* x, double* y, size_t n);
 
+/* The same product N x (APGD::performSchurComplementProduct, APGD.cu:81-87)
+ * by the kernels one pass of the graph-driven solver runs (build DV_APGD_FP64
+ * or DV_APGD_MIXED): J = F^T x stored per contact, the per-body gather
+ * tmp = M^-1 (sum_A J - sum_B J + bilateral rows) and the contact rows
+ * F (tmp_A - tmp_B), in that build's arithmetic (FP32 products for the mixed
+ * build).  Summed in another order than dv_schur_product, so it agrees with
+ * it to rounding.  The scene's state (gamma, v, p) is left unchanged. */
+int dv_pass_product(dv_scene* s, int build, const double* x, double* y, size_t n);
+
 /* Coulomb-cone projection of n 3-vectors in place with friction mu
  * (APGD.cu:47-78). */
 int dv_project_cone(dv_scene* s, double* g, size_t n, double mu);
@@ -238,6 +257,11 @@ typedef struct dv_batch_stats {
   double ms_assembly;       /* collision detection and assembly of all scenes */
   double ms_solve;          /* the shared APGD launches */
   double ms_update;
+  int rows_per_thread;      /* the solve's launch schedule, as in dv_stats */
+  int xcd_order;
+  int wg_per_cu_contacts;
+  int wg_per_cu_bodies;
+  int schedule_tuning;
 } dv_batch_stats;
 
 /* build: DV_APGD_FP64 or DV_APGD_MIXED (the graph-driven builds) */

@@ -39,6 +39,7 @@ def lib():
             "or_set_threads": ([ctypes.c_int], ctypes.c_int),
             "or_set_params": ([vp, _dp], None),
             "or_set_solver": ([vp, ctypes.c_int], ctypes.c_int),
+            "or_set_storage": ([vp, ctypes.c_int], ctypes.c_int),
             "or_set_bins": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], None),
             "or_add_bodies": ([vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _ip], ctypes.c_int),
             "or_add_bilateral": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double], ctypes.c_int),
@@ -82,7 +83,7 @@ class OracleSystem:
     """CPU restatement of System (System.cu) driven like the reference's drivers."""
 
     def __init__(self, scene, *, tol=None, max_iterations=None, h=None, mu=None,
-                 warm_start=None, anti_relax=False, bins=None, geom_mode=1, solver=APGD):
+                 warm_start=None, anti_relax=False, bins=None, geom_mode=1, solver=APGD, storage=0):
         L = lib()
         L.or_set_geom_mode(int(geom_mode))
         self._L = L
@@ -100,6 +101,8 @@ class OracleSystem:
         L.or_set_params(self._h, _d(a))
         if L.or_set_solver(self._h, int(solver)) != 0:
             raise ValueError("solver is APGD, PGJ or PGS")
+        if L.or_set_storage(self._h, int(storage)) != 0:
+            raise ValueError("storage is 0 (FP64), 1 (binary32 products) or 2 (binary32 products and iterates)")
         pos = np.ascontiguousarray(scene.pos, np.float64)
         vel = np.ascontiguousarray(scene.vel, np.float64)
         mass = np.ascontiguousarray(scene.mass, np.float64)

@@ -160,6 +160,13 @@ struct System {
   V3 gravity = mk(0, -9.81, 0);
   unsigned bins[3] = {20, 20, 20};
   bool warm = false, antiRelax = false;
+  // Precision study of APGD (not a reference option): 0 FP64 throughout; 1
+  // the Schur products N y, N gamma_new rounded to binary32 (the mixed GPU
+  // solver's products, its iterates FP64); 2 also the iterates gamma_new and
+  // y rounded to binary32.  Modes 1 and 2 take the Lipschitz test in the
+  // cancellation-free form D' N D >= L |D|^2 (D = gamma_new - y), as the
+  // mixed GPU solver does.
+  int storage = 0;
   double time = 0;
   int timeIndex = 0;
 
@@ -540,33 +547,44 @@ struct System {
     backtracks = 0;
     int it;
     std::vector<double> Ny, Ng;
-    for (it = 0; it < maxIt; ++it) {
-      schur(y, Ny);
-      axpby(Ny, r, g, 1.0, 1.0);
+    // precision study (storage 1, 2): binary32 rounding of products / iterates
+    auto round32 = [](std::vector<double>& x) {
+      for (double& e : x) e = (double)(float)e;
+    };
+    auto product = [&](const std::vector<double>& x, std::vector<double>& out) {
+      schur(x, out);
+      if (storage >= 1) round32(out);
+    };
+    auto step = [&]() {
       axpby(y, g, gammaNew, 1.0, -tt);
       project(gammaNew);
-      schur(gammaNew, Ng);
+      if (storage == 2) round32(gammaNew);
+      product(gammaNew, Ng);
+    };
+    // the Lipschitz test (APGD.cu:236-264), true when the step is rejected
+    auto rejected = [&]() {
+      axpby(gammaNew, y, t, 1.0, -1.0);
+      if (storage >= 1) return vdot(t, Ng) - vdot(t, Ny) >= L * vdot(t, t);
       obj1 = 0.5 * vdot(gammaNew, Ng) + vdot(gammaNew, r);
       obj2 = 0.5 * vdot(y, Ny) + vdot(y, r);
-      axpby(gammaNew, y, t, 1.0, -1.0);
-      double nr = vnrm2(t);
+      const double nr = vnrm2(t);
       obj2 += vdot(g, t) + 0.5 * L * (nr * nr);
-      while (obj1 >= obj2) {
+      return obj1 >= obj2;
+    };
+    for (it = 0; it < maxIt; ++it) {
+      product(y, Ny);
+      axpby(Ny, r, g, 1.0, 1.0);
+      step();
+      while (rejected()) {
         ++backtracks;
         L = 2.0 * L;
         tt = 1.0 / L;
-        axpby(y, g, gammaNew, 1.0, -tt);
-        project(gammaNew);
-        schur(gammaNew, Ng);
-        obj1 = 0.5 * vdot(gammaNew, Ng) + vdot(gammaNew, r);
-        obj2 = 0.5 * vdot(y, Ny) + vdot(y, r);
-        axpby(gammaNew, y, t, 1.0, -1.0);
-        nr = vnrm2(t);
-        obj2 += vdot(g, t) + 0.5 * L * (nr * nr);
+        step();
       }
       thetaNew = (-(theta * theta) + theta * std::sqrt(theta * theta + 4.0)) / 2.0;
       beta = theta * (1.0 - theta) / (theta * theta + thetaNew);
       axpby(gammaNew, gamma, yNew, 1.0 + beta, -beta);
+      if (storage == 2) round32(yNew);
       double res = getResidual(gammaNew, t);
       if (res < best) {
         best = res;
@@ -803,6 +821,13 @@ void or_set_params(void* sp, const double* a) {
 int or_set_solver(void* sp, int kind) {
   if (kind != 1 && kind != 5 && kind != 6) return -1;
   static_cast<System*>(sp)->solverKind = kind;
+  return 0;
+}
+
+// APGD precision study: 0 FP64, 1 binary32 products, 2 binary32 products and iterates
+int or_set_storage(void* sp, int mode) {
+  if (mode < 0 || mode > 2) return -1;
+  static_cast<System*>(sp)->storage = mode;
   return 0;
 }
 

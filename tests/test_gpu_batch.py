@@ -9,7 +9,9 @@ bar as well as the FP64 path.
 import numpy as np
 import pytest
 
+import oracle
 from dvide_amd import scenes, system
+from tests.helpers import rel
 
 pytestmark = pytest.mark.gpu
 
@@ -181,3 +183,55 @@ def test_server_rejects_a_second_request_in_flight(gpu):
 def test_batch_rejects_the_plain_solver(gpu):
     with pytest.raises(system.DvideError):
         system.Batch(system.BUILD_PLAIN)
+
+
+def _check_against_oracle(scene, g, tol, tag):
+    """One step of `g` from `scene`'s initial state against the oracle: the
+    residual the oracle evaluates in FP64 on the returned gamma, and the
+    free bodies' velocities against the oracle's solution at tolerance 1e-7
+    (as close as the oracle's own solution at `tol`, within a factor 3, or
+    2e-2 relative)."""
+    o = oracle.OracleSystem(scene, tol=tol, max_iterations=100000)
+    o.step()
+    exact = oracle.OracleSystem(scene, tol=1e-7, max_iterations=30000)
+    exact.step()
+    assert exact.residual < 1e-7, tag
+    st = g.stats()
+    assert st.num_contacts == o.num_contacts, tag
+    assert st.residual < tol, tag
+    o_asm = oracle.OracleSystem(scene, tol=tol)
+    o_asm.assemble()
+    assert o_asm.residual_of(g.gamma) < 1.1 * tol, tag
+    free = np.repeat(scene.fixed == 0, 3)
+    star = exact.vec(oracle.V)[free]
+    err = rel(g.v[free], star)
+    assert err < max(3 * rel(o.vec(oracle.V)[free], star), 1e-6) or err < 2e-2, (tag, err)
+
+
+@pytest.mark.parametrize("solver", [system.BUILD_FP64, system.BUILD_MIXED])
+def test_batch_members_match_oracle(gpu, solver):
+    """Each member of a batch step against the oracle, not only against its
+    own solve alone."""
+    tol = 1e-5
+    scs = [make(*c) for c in CASES]
+    members = [system.from_scene(sc, solver, tol=tol, max_iterations=50000) for sc in scs]
+    batch = system.Batch(solver)
+    for g in members:
+        batch.add(g)
+    batch.step()
+    for i, (sc, g) in enumerate(zip(scs, members)):
+        _check_against_oracle(sc, g, tol, i)
+    batch.close()
+
+
+def test_served_members_match_oracle(gpu):
+    tol = 1e-5
+    scs = [make(*c) for c in CASES]
+    members = [system.from_scene(sc, system.BUILD_MIXED, tol=tol, max_iterations=50000) for sc in scs]
+    srv = system.Server(system.BUILD_MIXED, max_slots=2)
+    for g in members:
+        srv.submit(g)
+    assert len(srv.drain()) == len(members)
+    for i, (sc, g) in enumerate(zip(scs, members)):
+        _check_against_oracle(sc, g, tol, i)
+    srv.close()

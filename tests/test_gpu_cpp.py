@@ -52,7 +52,9 @@ def test_cpp_one_ball(gpu, build, tmp_path):
     for i, j, v in rows:
         D[int(i) - 1, int(j) - 1] = float(v)
     blk = np.array([[0, -1, 0], [1, 0, 0], [0, 0, 1]], float)
-    np.testing.assert_array_equal(D, np.hstack([blk, -blk]))
+    # the solved steps leave the ball a few 1e-12 m off the box's axis, which
+    # tilts the normal by as much
+    np.testing.assert_allclose(D, np.hstack([blk, -blk]), rtol=0, atol=1e-9)
     _, dims, rows = read_mtx(tmp_path / "Minv.mtx")
     assert dims == [6, 6, 6]
     np.testing.assert_array_equal([float(v) for _, _, v in rows], [0, 0, 0, 1, 1, 1])

@@ -52,6 +52,23 @@ def test_full_size_contacts_assembly_and_product(gpu, workload):
     g.close()
 
 
+@pytest.mark.parametrize("solver,bound", [(system.BUILD_FP64, 1e-13), (system.BUILD_MIXED, 2e-6)])
+def test_full_size_pass_product(gpu, workload, solver, bound):
+    """The product the solvers iterate with (J = F^T x, P2's gather, P3's
+    rows) against the oracle's N x at the full size: FP64 to rounding, the
+    mixed build to binary32 rounding."""
+    o, g = both(workload, tol=TOL)
+    nc = o.assemble()
+    assert g.assemble() == nc and nc > 250000
+    x = np.random.default_rng(6).normal(size=3 * nc)
+    ref = o.schur(x)
+    y = g.pass_product(x, solver)
+    err = np.linalg.norm(y - ref) / np.linalg.norm(ref)
+    print(f"pass product, build {solver}: relative error {err:.2e} at {nc} contacts")
+    assert err < bound
+    g.close()
+
+
 @pytest.mark.parametrize("solver", [system.BUILD_FP64, system.BUILD_MIXED])
 def test_full_size_step(gpu, workload, solver):
     o, g = both(workload, solver=solver, tol=TOL, max_iterations=20000)

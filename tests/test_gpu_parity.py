@@ -191,21 +191,21 @@ def test_free_fall_gpu(gpu):
     assert np.allclose(g.p, p0 + h * g.v, rtol=0, atol=1e-15)
 
 
-@pytest.mark.parametrize("solver,bound", [(system.BUILD_FP64, 1e-10), (system.BUILD_MIXED, 1e-5)])
-def test_rows_per_thread_agree(gpu, solver, bound):
-    """One or two contacts per thread in the contact pass P1 only change the
-    order of the partial sums: the same ten iterations agree to rounding."""
+def test_rows_per_thread_agree(gpu):
+    """One or two contacts per thread in the contact passes P1 and P3 only
+    move rows between lanes: the partial sums keep the one-row-per-thread
+    order, so the same ten iterations are bit-identical (both builds)."""
     sc = scenes.pile(31, 6000)
-    out = []
-    for r in (1, 2):
-        g = system.from_scene(sc, solver, tol=0.0, max_iterations=10)
-        g.setRowsPerThread(r)
-        g.step()
-        out.append((g.stats().iterations, g.gamma.copy(), g.v.copy()))
-        g.close()
-    assert out[0][0] == out[1][0] == 10
-    assert rel(out[1][1], out[0][1]) < bound
-    assert rel(out[1][2], out[0][2]) < bound
+    for solver in (system.BUILD_FP64, system.BUILD_MIXED):
+        out = []
+        for r in (1, 2):
+            g = system.from_scene(sc, solver, tol=0.0, max_iterations=10)
+            g.setRowsPerThread(r)
+            g.step()
+            out.append((g.stats().iterations, g.gamma.copy(), g.v.copy()))
+            g.close()
+        assert out[0][0] == out[1][0] == 10
+        assert bits_equal(out[1][1], out[0][1]) and bits_equal(out[1][2], out[0][2]), solver
 
 
 def test_fp64_solver_velocities_match_pgs(gpu):
@@ -227,3 +227,73 @@ def test_fp64_solver_velocities_match_pgs(gpu):
     free = np.repeat(sc.fixed == 0, 3)
     vp = p.vec(oracle.V)[free]
     assert np.abs(g.v[free] - vp).max() < 1e-5 * np.abs(vp).max()
+
+
+@pytest.mark.parametrize("solver,bound", [(system.BUILD_FP64, 1e-13), (system.BUILD_MIXED, 2e-6)])
+def test_pass_product_matches_oracle(gpu, solver, bound):
+    """The product the graph-driven solvers iterate with -- J = F^T x per
+    contact, P2's per-body gather and P3's contact rows -- against the
+    oracle's COO product D (M^-1 (D^T x)) on a random x, bilateral rows
+    included: FP64 to rounding, the mixed build to binary32 rounding (the
+    frames, M^-1, J and tmp are binary32 there)."""
+    sc = scenes.compaction(2002, n=4000, width=1.5, lid=scenes.LID_BILATERAL)
+    o, g = both(sc)
+    nc = o.assemble()
+    assert g.assemble() == nc and nc > 1000
+    x = np.random.default_rng(11).normal(size=o.vec(oracle.R).size)
+    ref = o.schur(x)
+    y = g.pass_product(x, solver)
+    assert rel(y, ref) < bound, rel(y, ref)
+    assert np.abs(y - ref).max() < 10 * bound * np.abs(ref).max()
+    # the scene's own state is untouched: the step afterwards is the plain one
+    o.step()
+    g.step()
+    assert bits_equal(g.v, o.vec(oracle.V))
+
+
+@pytest.mark.parametrize("tol", [1e-4, 1e-6])
+def test_mixed_gamma_where_gamma_is_unique(gpu, tol):
+    """One layer of spheres on a floor (sliding and sticking), one contact
+    each: N is block diagonal and gamma is unique, so the mixed solver's
+    gamma must be within 1e-3 relative of the FP64 oracle's at the same
+    tolerance, and of the oracle's converged (1e-8) solution."""
+    sc = scenes.layer(7)
+    exact = oracle.OracleSystem(sc, tol=1e-8, max_iterations=100000)
+    exact.step()
+    assert exact.residual < 1e-8
+    o = oracle.OracleSystem(sc, tol=tol, max_iterations=100000)
+    o.step()
+    g = system.from_scene(sc, system.BUILD_MIXED, tol=tol, max_iterations=100000)
+    g.step()
+    st = g.stats()
+    assert st.num_contacts == o.num_contacts == 256
+    assert st.residual < tol
+    assert o.residual_of(g.gamma) < 1.1 * tol
+    assert rel(g.gamma, o.vec(oracle.GAMMA)) < 1e-3
+    assert rel(g.gamma, exact.vec(oracle.GAMMA)) < 1e-3
+    # and the FP64 graph-driven build to the oracle's own accuracy
+    f = system.from_scene(sc, system.BUILD_FP64, tol=1e-8, max_iterations=100000)
+    f.step()
+    assert rel(f.gamma, exact.vec(oracle.GAMMA)) < 1e-6
+
+
+def test_ccp_violation_matches_oracle(gpu):
+    """getCCPViolation (System.cu:1605-1643) of a bilateral-lid pile: on the
+    plain build, whose trajectory is the oracle's bit for bit, the four
+    values are bit-exact; on the graph-driven FP64 build they are the
+    oracle's evaluation of the same gamma and v, bit for bit."""
+    sc = scenes.compaction(2002, n=4000, width=1.5, lid=scenes.LID_BILATERAL)
+    o, g = both(sc, solver=system.BUILD_PLAIN, tol=1e-5, max_iterations=20000)
+    for _ in range(3):
+        o.step()
+        g.step()
+    viol = g.getCCPViolation()
+    assert bits_equal(viol, o.ccp_violation())
+    assert viol[0] <= 0 and viol[1] <= 0 and viol[3] >= 0
+    f = system.from_scene(sc, system.BUILD_FP64, tol=1e-5, max_iterations=20000)
+    f.step()
+    o1 = oracle.OracleSystem(sc, tol=1e-5, max_iterations=20000)
+    o1.step()
+    o1.set_vec(oracle.GAMMA, f.gamma)
+    o1.set_vec(oracle.V, f.v)
+    assert bits_equal(f.getCCPViolation(), o1.ccp_violation())
