@@ -12,6 +12,7 @@
 #include <cstring>
 #include <deque>
 #include <exception>
+#include <mutex>
 #include <new>
 #include <thread>
 
@@ -19,8 +20,34 @@ struct dv_scene {
   dv::Scene s;
 };
 
+// Staging of dv_schur_product_noalloc: x in the first kNoAllocRows doubles,
+// y in the second.  Every GPU the library runs on holds its own copy (16 MiB)
+// until the process ends.  In host code `scratch` names a host-side stand-in:
+// the device address of each GPU's copy is taken once per device, at scene
+// initialisation, and kept in scratchPtr.
+static constexpr size_t kNoAllocRows = size_t(1) << 20;
+static __device__ __align__(256) double scratch[2 * kNoAllocRows];
+
 namespace dv {
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex scratchMutex;
+double* scratchPtr[kMaxDevices] = {};
+
+void resolveScratch(int device) {
+  if (device < 0 || device >= kMaxDevices) throw DvError{DV_ERR_ARG, "device index beyond the scratch table"};
+  std::lock_guard<std::mutex> lock(scratchMutex);
+  if (scratchPtr[device]) return;
+  void* p = nullptr;
+  DV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));  // the current device's copy
+  scratchPtr[device] = static_cast<double*>(p);
+}
+
+double* scratchOf(int device) {
+  std::lock_guard<std::mutex> lock(scratchMutex);
+  return (device >= 0 && device < kMaxDevices) ? scratchPtr[device] : nullptr;
+}
 
 template <class F>
 int guarded(dv_scene* h, F&& fn) {
@@ -63,6 +90,7 @@ void initialize(Scene& s) {
   if (s.initialized) throw DvError{DV_ERR_STATE, "initializeSystem called twice"};
   DV_HIP(hipGetDevice(&s.device));
   DV_HIP(hipDeviceGetAttribute(&s.numCU, hipDeviceAttributeMultiprocessorCount, s.device));
+  resolveScratch(s.device);  // with s.device current
   if (!s.stream) DV_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
   for (auto& e : s.ev)
     if (!e) DV_HIP(hipEventCreate(&e));
@@ -683,6 +711,26 @@ int dv_pass_product(dv_scene* h, int build, const double* x, double* y, size_t n
     DV_HIP(hipMemcpyAsync(xs.p, x, sizeof(double) * n, hipMemcpyHostToDevice, s.stream));
     dv::passProduct(s, build == DV_APGD_MIXED, xs.p, y);
     xs.release();
+    return DV_OK;
+  });
+}
+
+int dv_schur_product_noalloc(dv_scene* h, const double* x, double* y, size_t n) {
+  return dv::guarded(h, [&](Scene& s) {
+    if (!x || !y) return dv::fail(s, DV_ERR_ARG, "null x or y");
+    if (n > kNoAllocRows)
+      return dv::fail(s, DV_ERR_ARG, "x and y exceed the 1,048,576 rows of the no-allocation scratch");
+    if (!s.initialized) return dv::fail(s, DV_ERR_STATE, "not initialized");
+    if (n != (size_t)s.R) return dv::fail(s, DV_ERR_ARG, "x and y need nbil + 3*nc entries");
+    if (n == 0) return DV_OK;
+    double* base = dv::scratchOf(s.device);  // this device's copy of `scratch`
+    if (!base) return dv::fail(s, DV_ERR_STATE, "scratch of the scene's device not resolved");
+    // x is overwritten in full and every row of y is written, so nothing in
+    // the scratch needs clearing
+    DV_HIP(hipMemcpyAsync(base, x, sizeof(double) * n, hipMemcpyHostToDevice, s.stream));
+    dv::schurExact(s, base, base + kNoAllocRows);
+    DV_HIP(hipMemcpyAsync(y, base + kNoAllocRows, sizeof(double) * n, hipMemcpyDeviceToHost, s.stream));
+    DV_HIP(hipStreamSynchronize(s.stream));
     return DV_OK;
   });
 }

@@ -33,7 +33,7 @@ EXPORTED = (
     "dv_scene_create", "dv_scene_destroy", "dv_last_error", "dv_add_bodies", "dv_add_bilateral_dof",
     "dv_set_bins_per_axis", "dv_set_param", "dv_get_param", "dv_initialize", "dv_step",
     "dv_num_bodies", "dv_get_bodies", "dv_num_contacts", "dv_get_stats", "dv_get_vector", "dv_set_vector",
-    "dv_get_contacts", "dv_ccp_violation", "dv_assemble", "dv_schur_product",
+    "dv_get_contacts", "dv_ccp_violation", "dv_assemble", "dv_schur_product", "dv_schur_product_noalloc",
     "dv_pass_product", "dv_project_cone", "dv_solve", "dv_get_jacobian", "dv_contact_forces", "dv_energies",
     "dv_batch_create", "dv_batch_destroy", "dv_batch_last_error", "dv_batch_add", "dv_batch_step",
     "dv_batch_get_stats",
@@ -122,6 +122,7 @@ def load_library(path: str = LIB_PATH):
         "dv_ccp_violation": ([vp, _dp], ctypes.c_int),
         "dv_assemble": ([vp, _ip], ctypes.c_int),
         "dv_schur_product": ([vp, _dp, _dp, ctypes.c_size_t], ctypes.c_int),
+        "dv_schur_product_noalloc": ([vp, _dp, _dp, ctypes.c_size_t], ctypes.c_int),
         "dv_pass_product": ([vp, ctypes.c_int, _dp, _dp, ctypes.c_size_t], ctypes.c_int),
         "dv_project_cone": ([vp, _dp, ctypes.c_size_t, ctypes.c_double], ctypes.c_int),
         "dv_solve": ([vp], ctypes.c_int),
@@ -501,6 +502,15 @@ class System:
         y = np.zeros_like(x)
         self._check(self._L.dv_pass_product(self._h, int(build), x.ctypes.data_as(_dp), y.ctypes.data_as(_dp),
                                             x.size))
+        return y
+
+    def schur_product_noalloc(self, x):
+        """N x through the library's fixed device scratch (no allocation;
+        at most 1,048,576 rows)."""
+        x = _f64(x)
+        y = np.zeros_like(x)
+        self._check(self._L.dv_schur_product_noalloc(self._h, x.ctypes.data_as(_dp), y.ctypes.data_as(_dp),
+                                                     x.size))
         return y
 
 

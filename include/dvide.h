@@ -230,6 +230,18 @@ int dv_schur_product(dv_scene* s, const double* x, double* y, size_t n);
  * it to rounding.  The scene's state (gamma, v, p) is left unchanged. */
 int dv_pass_product(dv_scene* s, int build, const double* x, double* y, size_t n);
 
+/* dv_schur_product without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  x and y are staged through a fixed device array of the
+ * library (16 MiB per GPU, held until the process ends) instead of a buffer
+ * allocated per call, so n is at most 1,048,576 rows (about 349k contacts);
+ * larger n fails with DV_ERR_ARG.  The array is shared by every call on a
+ * GPU and keeps the last call's contents: calls that overlap on one GPU
+ * (from several host threads or scenes) race and must be serialised by the
+ * caller.  Results are those of dv_schur_product, bit for bit. */
+int dv_schur_product_noalloc(dv_scene* s, const double* x, double* y, size_t n);
+
 /* Coulomb-cone projection of n 3-vectors in place with friction mu
  * (APGD.cu:47-78). */
 int dv_project_cone(dv_scene* s, double* g, size_t n, double mu);
