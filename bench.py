@@ -84,6 +84,7 @@ def run_gpu(scene, solver, steps, warmup, profile_kernel=0, max_iterations=None)
         g.setProfileKernel(profile_kernel)
         hip.hipDeviceSynchronize()
         lat, its, res, nc, passes, solve, kms, kcnt, kstep = [], [], [], [], [], [], 0.0, 0, []
+        any_tuning = False
         t0 = time.perf_counter()
         for _ in range(steps):
             ts = time.perf_counter()
@@ -98,6 +99,7 @@ def run_gpu(scene, solver, steps, warmup, profile_kernel=0, max_iterations=None)
             kms += st.prof_kernel_ms
             kcnt += st.prof_kernel_count
             kstep.append(round(1e3 * st.prof_kernel_ms / st.prof_kernel_count, 3) if st.prof_kernel_count else None)
+            any_tuning = any_tuning or bool(st.schedule_tuning)
         hip.hipDeviceSynchronize()
         elapsed = time.perf_counter() - t0
         last = g.stats()
@@ -108,6 +110,8 @@ def run_gpu(scene, solver, steps, warmup, profile_kernel=0, max_iterations=None)
             phases=dict(collision=last.ms_collision, assembly=last.ms_assembly, solve=last.ms_solve,
                         update=last.ms_update),
             nb=g.numBodies, v=g.v,
+            schedule=dict(rows_per_thread=last.rows_per_thread, xcd_order=last.xcd_order,
+                          measured_in_timed_steps=int(any_tuning)),
         )
     finally:
         g.close()
@@ -250,6 +254,10 @@ def batched(n_scenes=512, steps=2, settle=3, seed=4004):
         "mean_iterations": round(float(np.mean([s.total_iterations / n_scenes for s in stats])), 1),
         "ms_assembly": round(float(np.mean([s.ms_assembly for s in stats])), 2),
         "ms_solve": round(float(np.mean([s.ms_solve for s in stats])), 2),
+        "schedule": {"rows_per_thread": stats[-1].rows_per_thread, "xcd_order": stats[-1].xcd_order,
+                     "wg_per_cu_contacts": stats[-1].wg_per_cu_contacts,
+                     "wg_per_cu_bodies": stats[-1].wg_per_cu_bodies,
+                     "measured_in_timed_steps": int(any(x.schedule_tuning for x in stats))},
         "settle_batch_steps": settle,
         "settle_s": round(settle_s, 1),
         "scene_generation_s": round(gen, 1),
@@ -513,6 +521,9 @@ def main():
     ap.add_argument("--only-mixed", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-scene batched solve")
     ap.add_argument("--no-serve", action="store_true", help="skip the served request stream")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run only the batched leg (with --batch-scenes scenes) and print its figures")
+    ap.add_argument("--batch-scenes", type=int, default=512)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB copy kernel (for scaling rocprofv3 byte counters)")
     ap.add_argument("--summarize", metavar="CSV",
@@ -531,6 +542,10 @@ def main():
 
     from dvide_amd import scenes, system
 
+    if args.only_batch:
+        progress(f"{args.batch_scenes}-scene batch")
+        print(json.dumps({"batched": batched(n_scenes=args.batch_scenes)}), flush=True)
+        return
     progress("settling the compaction pile")
     scene, settle_its = settled_scene(args.settle)
     progress("mixed solver")
@@ -596,6 +611,7 @@ def main():
         "residual_max": float(np.max(mixed["residual"])),
         "steps_converged": int(np.sum(mixed["residual"] < TOL)),
         "phases_ms_last_step": {k: round(v, 3) for k, v in mixed["phases"].items()},
+        "schedule": mixed["schedule"],
         "roofline": {
             "bound": "hbm",
             "kernel": "kP1<float> (contact pass: previous decision, momentum, projection, F^T gamma)",
@@ -659,8 +675,8 @@ def main():
         progress("lid-pressed leg")
         line["lid_pressed"] = lid_pressed()
     if not args.no_batch and not args.only_mixed:
-        progress("512-scene batch")
-        line["batched"] = batched()
+        progress(f"{args.batch_scenes}-scene batch")
+        line["batched"] = batched(n_scenes=args.batch_scenes)
     if not args.no_serve and not args.only_mixed:
         progress("served stream")
         line["served"] = served()
