@@ -41,16 +41,30 @@ def _hip():
     return lib
 
 
-def p1_bytes_per_contact(mixed: bool) -> int:
+def p1_bytes_per_contact(mixed: bool, fresh: bool = True) -> int:
     """Algorithmic bytes the contact pass P1 moves per contact (3 rows).
 
-    Reads gamma_k, gamma_{k-1} (FP64), N gamma_k, N gamma_{k-1} and r (TM),
-    the stored frame (FP64 n, or FP32 n and u); writes gamma_new (FP64), the
-    world-frame impulse J = F^T gamma_new (3 TM) and, in the mixed solver,
-    delta = gamma_new - y (3 TM).
+    Reads gamma_k, gamma_{k-1} (FP64), N gamma_{k-1} and r (TM), the stored
+    frame (FP64 n, or FP32 n and u); writes gamma_new (FP64) and the
+    world-frame impulse J = F^T gamma_new (3 TM).  N gamma_k: when the
+    previous pass accepted gamma_k (fresh), P1 forms it from the contact's two
+    body ids and the bodies' M^-1 D^T gamma_k (3 TM each) and stores it (3
+    TM); otherwise it reads it back (3 TM).  J's second copy in the B-side
+    order (and the index it is stored at) is a layout choice and is not
+    counted.
     """
     s = 4 if mixed else 8
-    return 2 * 24 + 9 * s + 24 + 24 + 3 * s + (3 * s if mixed else 0)
+    n_gamma_k = (8 + 6 * s + 3 * s) if fresh else 3 * s
+    return 2 * 24 + 6 * s + 24 + n_gamma_k + 24 + 3 * s
+
+
+def p2_bytes(nc: int, nb: int, mixed: bool) -> int:
+    """Algorithmic bytes of the body pass P2: each contact's J read by both
+    of its bodies (3 TM each time); per body its three index offsets, M^-1,
+    the store of M^-1 D^T gamma_new (3 TM) and, in the mixed solver, the
+    D^T gamma ring (two reads and one store of 3 TM) for the Lipschitz term."""
+    s = 4 if mixed else 8
+    return nc * 6 * s + nb * (12 + s + 3 * s + (9 * s if mixed else 0))
 
 
 SETTLE_STEPS = 30
@@ -552,13 +566,12 @@ def main():
     cap = args.max_iterations
     mixed = run_gpu(scene, system.BUILD_MIXED, args.steps, args.warmup, profile_kernel=system.PROFILE_P1,
                     max_iterations=cap)
-    # burst-timed averages of the body pass P2 and the contact pass P3 (two
-    # untimed steps each from the same state): the Schur-product phase
-    kernel_us = {"P1": None, "P2": None, "P3": None}
-    for pk, name in ((system.PROFILE_P2, "P2"), (system.PROFILE_P3, "P3")):
-        r = run_gpu(scene, system.BUILD_MIXED, 2, 0, profile_kernel=pk, max_iterations=cap)
-        if r["kernel_count"]:
-            kernel_us[name] = round(1e3 * r["kernel_ms"] / r["kernel_count"], 3)
+    # burst-timed average of the body pass P2 (two untimed steps from the
+    # same state); P1 + P2 is one solver pass
+    kernel_us = {"P1": None, "P2": None}
+    r = run_gpu(scene, system.BUILD_MIXED, 2, 0, profile_kernel=system.PROFILE_P2, max_iterations=cap)
+    if r["kernel_count"]:
+        kernel_us["P2"] = round(1e3 * r["kernel_ms"] / r["kernel_count"], 3)
     others = {}
     if not args.only_mixed:
         progress("FP64 graph-driven and plain FP64 solvers")
@@ -569,7 +582,11 @@ def main():
     value = args.steps / mixed["elapsed"]
     nc = int(np.mean(mixed["contacts"]))
     k_us = 1e3 * mixed["kernel_ms"] / max(mixed["kernel_count"], 1)
-    alg = p1_bytes_per_contact(True) * nc
+    # a pass after an accepted iterate forms N gamma_k (fresh), one after a
+    # backtrack reads it: P1's bytes weighted by the share of fresh passes
+    fresh = float(np.sum(mixed["iterations"])) / max(float(np.sum(mixed["passes"])), 1.0)
+    b_p1 = fresh * p1_bytes_per_contact(True, True) + (1 - fresh) * p1_bytes_per_contact(True, False)
+    alg = b_p1 * nc
     achieved = alg / (k_us * 1e-6) / 1e9 if k_us > 0 else 0.0
     # HBM bytes per P1 launch: the per-contact figure of the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE summary (scaled by a copy kernel) times this
@@ -623,22 +640,25 @@ def main():
             "avg_us": round(k_us, 3),
             "launches_timed": mixed["kernel_count"],
             "avg_us_per_step": mixed["kernel_us_steps"],
-            "bytes_per_contact": p1_bytes_per_contact(True),
+            "bytes_per_contact": round(b_p1, 1),
+            "fresh_pass_share": round(fresh, 4),
         },
     }
     kernel_us["P1"] = round(k_us, 3)
     line["kernel_us"] = kernel_us
-    if kernel_us["P2"] and kernel_us["P3"]:
-        # the algorithmic bytes of one Schur product N x with a 3x3 frame per
-        # contact, mixed precision: 112 B per contact + 28 B per body
-        b_n = 112 * nc + 28 * mixed["nb"]
-        t_n = kernel_us["P2"] + kernel_us["P3"]
-        line["spmv_phase"] = {
-            "kernels": "kBody (D^T gather, M^-1) + kP3 (D gather, Delta . N gamma_new)",
-            "bytes": b_n,
-            "us": round(t_n, 3),
-            "achieved_gbs": round(b_n / (t_n * 1e-6) / 1e9, 1),
-            "frac": round(b_n / (t_n * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+    if kernel_us["P2"]:
+        # one solver pass: P1 (the vector algebra, the stores of gamma_new and
+        # F^T gamma_new, and the rows of N gamma_k) and P2 (the per-body
+        # gather M^-1 D^T gamma_new and the Lipschitz term); the Schur product
+        # is spread over the two
+        b_pass = alg + p2_bytes(nc, mixed["nb"], True)
+        t_pass = k_us + kernel_us["P2"]
+        line["pass_phase"] = {
+            "kernels": "kP1 + kBody (one APGD pass, the Schur product included)",
+            "bytes": int(round(b_pass)),
+            "us": round(t_pass, 3),
+            "achieved_gbs": round(b_pass / (t_pass * 1e-6) / 1e9, 1),
+            "frac": round(b_pass / (t_pass * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
         }
     def us_per_iteration(r):
         return round(1e3 * float(np.sum(r["solve_ms"])) / max(int(np.sum(r["iterations"])), 1), 2)

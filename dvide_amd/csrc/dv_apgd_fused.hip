@@ -4,16 +4,20 @@
 // launches.
 //
 // One solver pass (one Schur product plus the vector algebra around it) is
-// three kernels over all active scenes:
-//   P1 (contacts): y and N y from the last two iterates, g = N y + r,
+// two kernels over all active scenes:
+//   P1 (contacts): N gamma_k = F (tmp_A - tmp_B) from the previous pass's
+//       tmp when the previous decision accepted gamma_k (else read back),
+//       y and N y from the last two iterates, g = N y + r,
 //       gamma_new = Pi(y - t g), the dot products of the objective and
 //       restart tests, J_c = F_c^T gamma_new (the world-frame impulse of
 //       contact c on body A), and the infinity norm of getResidual
 //       (APGD.cu:90-102) of the iterate the previous decision accepted;
-//   P2 (bodies):   tmp_b = M_b^-1 (sum_A J_c - sum_B J_c + bilateral rows);
-//       bodies with many contacts are reduced by a whole workgroup;
-//   P3 (contacts): (N gamma_new)_c = F_c (tmp_A - tmp_B) and gamma_new .
-//       N gamma_new (FP64) or Delta . N gamma_new (mixed);
+//   P2 (bodies):   s_b = sum_A J_c - sum_B J_c + bilateral rows (D^T
+//       gamma_new), tmp_b = M_b^-1 s_b, and the Lipschitz test's body terms:
+//       s . tmp, whose sum is gamma_new . N gamma_new (FP64), or, with the
+//       ring of per-body D^T gamma, d . M^-1 d for d = D^T (gamma_new - y),
+//       whose sum is Delta' N Delta (mixed); bodies with many contacts are
+//       reduced by a whole workgroup;
 // followed by each scene's decision: a fixed-order reduction of its partial
 // sums, the Lipschitz test (APGD.cu:236-264) and either a backtracking step
 // or the acceptance logic (APGD.cu:266-332).  Several scenes decide in a
@@ -54,7 +58,7 @@
 // The mixed solver evaluates the Lipschitz test in the cancellation-free form
 // D'N D >= L |D|^2 (D = gamma_new - y), equal to obj1 >= obj2 for g = N y + r
 // and symmetric N, because its binary32 N gamma would swamp the small
-// difference obj2 - obj1.
+// difference obj2 - obj1; D'N D = |M^-1/2 D^T D|^2 is summed over the bodies.
 #include "dv_geom.h"
 #include "dv_internal.h"
 
@@ -76,6 +80,7 @@ struct ApgdState {
   int cur, prev, nxt;
   int hatSrc, done, converged, passes;
   int pendingRes;  // the iterate in slot cur was accepted and its residual is still to be taken
+  int fresh;       // N of ring slot cur is still to be formed (by the next P1, from tmp)
 };
 
 // What the pass kernels read of a slot's state, written beside the state by
@@ -83,15 +88,15 @@ struct ApgdState {
 // one 16-byte load; the ring pointers follow by arithmetic from the slot's
 // ring base, so a pass kernel's data loads wait on a single round trip.
 struct alignas(32) PassView {
-  int cur, prev, nxt, flags;  // flags: 1 done, 2 restart
+  int cur, prev, nxt, flags;  // flags: 1 done, 2 restart, 4 fresh
   double t, beta;
 };
-constexpr int kViewDone = 1, kViewRestart = 2;
+constexpr int kViewDone = 1, kViewRestart = 2, kViewFresh = 4;
 
 __device__ __forceinline__ PassView viewOf(const ApgdState& S) {
   PassView v;
   v.cur = S.cur; v.prev = S.prev; v.nxt = S.nxt;
-  v.flags = (S.done ? kViewDone : 0) | (S.restart ? kViewRestart : 0);
+  v.flags = (S.done ? kViewDone : 0) | (S.restart ? kViewRestart : 0) | (S.fresh ? kViewFresh : 0);
   v.t = S.t; v.beta = S.beta;
   return v;
 }
@@ -175,12 +180,14 @@ struct Args {
   V4<TM>* JB;           // the same, in B-side order: body b's B incidences are
                         // [iStart[b] - cStart[b], iStart[b+1] - cStart[b+1])
   V4<TM>* tmp;
+  V4<TM>* sr;           // mixed solver: D^T of each ring slot's iterate, per body (slot k at k * nb)
   double* gHat;         // gammaHat before any iterate is accepted (ones)
-  TM* delta;            // mixed solver: gamma_new - y
   double* part1[2];     // kSlots per tile, by pass parity
   double* part3[2];
-  double* q3[2];        // P3's one partial per tile (slot 6), packed, by pass parity
+  double* lq[2];        // P2's Lipschitz partials, by pass parity: one per body tile, then one
+                        // per heavy body (in the heavy list's order)
   double* p1sum[2];     // single slot: P1's slots totalled by P2's last workgroup
+  int nBodyTiles;       // body tiles of the scene (kTB bodies each)
 };
 
 // Launch-wide tables (device memory, so that a captured graph stays valid
@@ -189,6 +196,7 @@ struct Tables {
   const Tile* items;    // row tiles (kTC rows of one slot)
   const Tile* bodies;   // body tiles (kTB bodies of one slot)
   const int2* heavy;    // (slot, body) of bodies reduced by a workgroup
+  const int* heavyIdx;  // the index of each in its slot's heavy list
   const int* counts;    // [0] row tiles, [1] body tiles, [2] heavy bodies
   const int* slots;     // the set's slots (one decide workgroup each)
   ApgdState* st;        // [slot][2]: the solver state by pass parity
@@ -348,7 +356,7 @@ __device__ void reduceAll(const double* part1, const double* part3, int ng, doub
 // loads of eight tiles issued together), each wavefront combines its lanes by
 // a butterfly, and the wavefront totals are added in wavefront order.  out[]
 // holds the NS totals in every thread.  The pass decision reads P1's slots
-// and P3's slot 6 this way, wherever it takes them from.  In a wider
+// and P2's Lipschitz partials this way, wherever it takes them from.  In a wider
 // workgroup the threads past kTP add nothing, so the totals are the same.
 constexpr int kTP = 256;
 template <int NS, int kStride = kSlots>
@@ -392,28 +400,28 @@ __device__ void reduceSlots(const double* part, int s0, int ng, double (&out)[NS
 }
 
 // The totals the pass decision needs: P1's slots 0..5 and 7 (slot 6 is
-// unused) and P3's slot 6 (its one partial per tile, packed in q3); the
-// other slots are 0.  reducePass reduces P1's partials; passTotals takes
-// them already totalled (a single slot's P2 totals them), which keeps the
-// registers of the eight-slot reduction out of the kernels that decide a
-// single slot.
-__device__ void reducePass(const double* part1, const double* q3, int ng, double* out) {
+// unused) and, in slot kSlots + 6, P2's Lipschitz term (its partials per
+// body tile and per heavy body, packed in lq, nl of them); the other slots
+// are 0.  reducePass reduces P1's partials; passTotals takes them already
+// totalled (a single slot's P2 totals them), which keeps the registers of
+// the eight-slot reduction out of the kernels that decide a single slot.
+__device__ void reducePass(const double* part1, int ng, const double* lq, int nl, double* out) {
 #pragma unroll
   for (int s = 0; s < 2 * kSlots; ++s) out[s] = 0.0;
   double t1[kSlots], t3[1];
   reduceSlots<kSlots>(part1, 0, ng, t1);
-  reduceSlots<1, 1>(q3, 0, ng, t3);
+  reduceSlots<1, 1>(lq, 0, nl, t3);
 #pragma unroll
   for (int s = 0; s < kSlots; ++s) out[s] = t1[s];
   out[kSlots + 6] = t3[0];
 }
-__device__ void passTotals(const double* p1, const double* q3, int ng, double* out) {
+__device__ void passTotals(const double* p1, const double* lq, int nl, double* out) {
 #pragma unroll
   for (int s = 0; s < 2 * kSlots; ++s) out[s] = 0.0;
   double t3[1];
 #pragma unroll
   for (int s = 0; s < kSlots; ++s) out[s] = gl(p1)[s];
-  reduceSlots<1, 1>(q3, 0, ng, t3);
+  reduceSlots<1, 1>(lq, 0, nl, t3);
   out[kSlots + 6] = t3[0];
 }
 
@@ -469,6 +477,17 @@ __device__ __forceinline__ TM bilRow(const Args<TM>& a, int q) {
   TM s = tmpDof(a, gl(a.bilA)[q]);
   if (gl(a.bilB)[q] >= 0) s -= tmpDof(a, gl(a.bilB)[q]);
   return s;
+}
+
+// (N x)_c = F_c (tmp_A - tmp_B): the three rows of a contact from the two
+// bodies' M^-1 D^T x
+template <typename TM>
+__device__ __forceinline__ void rowsOfTmp(const TM (&F)[9], const V4<TM>& tA, const V4<TM>& tB, TM& n0, TM& n1,
+                                          TM& n2) {
+  const TM d0 = tA.x - tB.x, d1 = tA.y - tB.y, d2 = tA.z - tB.z;
+  n0 = F[0] * d0 + F[1] * d1 + F[2] * d2;
+  n1 = F[3] * d0 + F[4] * d1 + F[5] * d2;
+  n2 = F[6] * d0 + F[7] * d1 + F[8] * d2;
 }
 
 template <typename TM>
@@ -606,8 +625,33 @@ __device__ __forceinline__ TM bodyMinv(const Args<TM>& a, int g, int which, bool
   return gl(a.minv)[g];
 }
 
+// The Lipschitz test's term of one body in the pass whose view is S, from
+// the body's D^T gamma_new (s) and M^-1: FP64, s . (M^-1 s), whose sum over
+// the bodies is gamma_new' N gamma_new; mixed, d . (M^-1 d) with d = s - D^T y
+// and D^T y = (1 + beta) D^T gamma_k - beta D^T gamma_{k-1} from the ring of
+// per-body D^T gamma (gamma_k on a restart), whose sum is Delta' N Delta for
+// Delta = gamma_new - y.  The mixed solver keeps s in ring slot nxt.
+template <typename TM>
+__device__ __forceinline__ double lipschitzTerm(const Args<TM>& a, int g, const PassView& S, TM mi, TM s0, TM s1,
+                                                TM s2) {
+  constexpr bool kMixed = sizeof(TM) < sizeof(double);
+  if (!kMixed) return (double)s0 * (double)(mi * s0) + (double)s1 * (double)(mi * s1) + (double)s2 * (double)(mi * s2);
+  const long nb = a.dims.nb;
+  const V4<TM> c = ld4(a.sr, S.cur * nb + g), p = ld4(a.sr, S.prev * nb + g);
+  V4<TM> sn;
+  sn.x = s0; sn.y = s1; sn.z = s2; sn.w = 0;
+  st4(a.sr, S.nxt * nb + g, sn);
+  const bool mom = !(S.flags & kViewRestart);
+  const double bp = 1.0 + S.beta, bm = -S.beta;
+  const double d0 = (double)s0 - (mom ? bp * (double)c.x + bm * (double)p.x : (double)c.x);
+  const double d1 = (double)s1 - (mom ? bp * (double)c.y + bm * (double)p.y : (double)c.y);
+  const double d2 = (double)s2 - (mom ? bp * (double)c.z + bm * (double)p.z : (double)c.z);
+  return (double)mi * (d0 * d0 + d1 * d1 + d2 * d2);
+}
+
 template <typename TM, int which>
 __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tables T, int sIdx) {
+  constexpr bool kMixed = sizeof(TM) < sizeof(double);
   static_assert(kTB == kTP, "P2's extra workgroup reduces P1's partials like the decision");
   // the body workgroups (a single slot's pass has one more, the last)
   const int nw = (which == 2 && T.single) ? (int)gridDim.x - 1 : (int)gridDim.x;
@@ -676,6 +720,19 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
         }
       }
       out.x = mi * s0; out.y = mi * s1; out.z = mi * s2;
+      if (which == 2) {
+        // this body's Lipschitz term; the tile's sum goes to lq (bodies reduced
+        // by a workgroup add theirs below)
+        double term[1] = {own && !heavy ? lipschitzTerm(a, gs, S, mi, s0, s1, s2) : 0.0};
+        __shared__ double tt[1];
+        blockReduce<kTB, 1>(term, tt);
+        if (threadIdx.x == 0) gl(a.lq[sIdx ^ 1])[tile.start / kTB] = tt[0];
+      }
+      if (which == 1 && kMixed && own && !heavy) {  // D^T gamma_0 into ring slot 0
+        V4<TM> sv;
+        sv.x = s0; sv.y = s1; sv.z = s2; sv.w = 0;
+        st4(a.sr, g, sv);
+      }
     }
     if (own && !heavy) st4(a.tmp, g, out);
   };
@@ -703,11 +760,14 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
     const Args<TM>& a = A[ref.x];
     const int g = ref.y;
     const TM mi = bodyMinv(a, g, which);
+    // this body's slot among the Lipschitz partials, after the body tiles'
+    double* lqSlot = which == 2 ? a.lq[sIdx ^ 1] + a.nBodyTiles + T.heavyIdx[hq] : nullptr;
     if (mi == (TM)0) {
       if (threadIdx.x == 0) {
         V4<TM> out;
         out.x = 0; out.y = 0; out.z = 0; out.w = 0;
         st4(a.tmp, g, out);
+        if (which == 2) gl(lqSlot)[0] = 0.0;
       }
       continue;
     }
@@ -741,6 +801,12 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
       out.z = mi * (TM)tot[2];
       out.w = 0;
       st4(a.tmp, g, out);
+      if (which == 2) gl(lqSlot)[0] = lipschitzTerm(a, g, S, mi, (TM)tot[0], (TM)tot[1], (TM)tot[2]);
+      if (which == 1 && kMixed) {
+        V4<TM> sv;
+        sv.x = (TM)tot[0]; sv.y = (TM)tot[1]; sv.z = (TM)tot[2]; sv.w = 0;
+        st4(a.sr, g, sv);
+      }
     }
   }
 }
@@ -819,6 +885,7 @@ __global__ __launch_bounds__(kTD) void kSetupDecide(const Args<TM>* __restrict__
     S.it = 0; S.bt = 0; S.backtracks = 0; S.restart = 1;
     S.cur = 0; S.prev = 0; S.nxt = 1; S.hatSrc = -1;  // -1: gHat (ones)
     S.pendingRes = 0;  // gamma_0 is not an accepted iterate (APGD.cu:228-230)
+    S.fresh = 0;       // N gamma_0 is in rn[0] (kSetup5)
     S.done = a.dims.maxIt <= 0 ? 1 : 0;
     S.converged = 0;
     S.passes = 0;
@@ -838,11 +905,11 @@ template <bool kMixed>
 __device__ void decideStep(ApgdState& S, const double* tot, const SolveDims& D) {
   for (int k = 0; k < 2 * kSlots; ++k) S.sum[k] = tot[k];
   const double* s1 = tot;           // P1 slots
-  const double* s3 = tot + kSlots;  // P3 slots
+  const double* s3 = tot + kSlots;  // slot 6: P2's Lipschitz term
   S.passes++;
   // getResidual of the iterate the previous decision accepted (now in slot
   // cur), evaluated by this pass's P1, then gammaHat and the stopping test
-  // (APGD.cu:276-289).  Taking it one pass late spares P3 the iterate and r;
+  // (APGD.cu:276-289).  Taking it one pass late spares a pass the reload of the iterate and r;
   // the decisions and the result are those of the reference's order.
   if (S.pendingRes) {
     S.pendingRes = 0;
@@ -870,7 +937,8 @@ __device__ void decideStep(ApgdState& S, const double* tot, const SolveDims& D) 
   const double L = S.L;
   bool backtrack;
   if (kMixed) {
-    backtrack = (s3[6] - s1[1]) >= L * s1[4];
+    // P2's body sums give Delta' N Delta = |M^-1/2 D^T Delta|^2 directly
+    backtrack = s3[6] >= L * s1[4];
   } else {
     const double obj1 = 0.5 * s3[6] + s1[0];
     const double nrm = sqrt(s1[4]);
@@ -883,6 +951,7 @@ __device__ void decideStep(ApgdState& S, const double* tot, const SolveDims& D) 
     S.backtracks++;
     S.L = 2.0 * L;
     S.t = 1.0 / S.L;
+    S.fresh = 0;  // this pass's P1 has formed and stored N of slot cur
     return;
   }
   const double theta = S.theta;
@@ -905,6 +974,7 @@ __device__ void decideStep(ApgdState& S, const double* tot, const SolveDims& D) 
   S.it++;
   S.bt = 0;
   S.pendingRes = 1;
+  S.fresh = 1;  // gamma_new's N is formed by the next P1, from the tmp of this pass's P2
 }
 
 // ---------------------------------------------------------------------------
@@ -933,14 +1003,16 @@ __device__ __forceinline__ void frameFrom(const float (&c)[6], int, float (&F)[9
 // loads of all its rows before computing, for more memory-level parallelism
 // per wavefront.  The per-tile partial sums keep their layout.
 //
-// P1 with one row per thread is held to 80 VGPRs (six wavefronts per SIMD;
-// the FP64 instance spills a few registers to scratch), so that a CU keeps three
-// of its 512-thread workgroups in flight rather than two: past two tiles per
-// CU (262,144 rows on 256 CUs), the CUs given a third tile then run it beside
-// the other two instead of after one of them (P1 11% faster at 263k contacts,
-// 2% slower at 256k, where no CU takes a third tile).
-template <int R>
-constexpr int kP1MinWaves = R == 1 ? 6 : 1;
+// The mixed P1 with one row per thread is held to 80 VGPRs (six wavefronts
+// per SIMD), so that a CU keeps three of its 512-thread workgroups in flight
+// rather than two: past two tiles per CU (262,144 rows on 256 CUs), the CUs
+// given a third tile then run it beside the other two instead of after one
+// of them (P1 11% faster at 263k contacts, 2% slower at 256k, where no CU
+// takes a third tile).  The FP64 instance, which holds twice the registers
+// per value and, with the fresh iterate's N formed here, needs 112 VGPRs, is
+// held to four wavefronts so that it does not spill.
+template <typename TM, int R>
+constexpr int kP1MinWaves = R == 1 ? (sizeof(TM) < sizeof(double) ? 6 : 4) : 1;
 
 // The decision of the previous pass (parity par ^ 1) of a single slot, taken
 // by every workgroup of the next pass's P1 instead of by a kernel of its own:
@@ -953,7 +1025,7 @@ __device__ PassView decidePrevious(const Args<TM>& a, const Tables& T, int par) 
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   ApgdState S = STATE(T, 0, par);
   double tot[2 * kSlots];
-  passTotals(a.p1sum[par ^ 1], a.q3[par ^ 1], a.nTiles, tot);
+  passTotals(a.p1sum[par ^ 1], a.lq[par ^ 1], a.nBodyTiles + gl(a.counters)[2], tot);
   if (!S.done) decideStep<kMixed>(S, tot, a.dims);
   const PassView v = viewOf(S);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -966,7 +1038,7 @@ __device__ PassView decidePrevious(const Args<TM>& a, const Tables& T, int par) 
 
 // kFold: a single slot's P1 takes the previous pass's decision first.
 template <typename TM, int R, bool kFold>
-__global__ __launch_bounds__(kTC / R, kP1MinWaves<R>) void kP1(const Args<TM>* __restrict__ A, Tables T, int par) {
+__global__ __launch_bounds__(kTC / R, (kP1MinWaves<TM, R>)) void kP1(const Args<TM>* __restrict__ A, Tables T, int par) {
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   constexpr int NT = kTC / R;
   constexpr int NF = kMixed ? 6 : 3;  // stored frame components
@@ -983,27 +1055,73 @@ __global__ __launch_bounds__(kTC / R, kP1MinWaves<R>) void kP1(const Args<TM>* _
     // rows the thread does not own read row 0 instead (always present), so
     // that no load is conditional and no select waits on one; their values
     // are never used
+    // N gamma_k: formed here from the tmp of the previous pass's P2 when the
+    // previous decision accepted gamma_k (fresh), and stored for the passes
+    // after; read back otherwise (the flag is the same in every workgroup)
+    const bool fresh = (S.flags & kViewFresh) != 0;
     double gk[R][3], gm[R][3];
     TM rv[R][3], nk[R][3], nm[R][3], fc[R][6];
     int pb[R];  // the contacts' B-side slots, loaded with the rest
+    int ia[R], ib[R];
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       const long q = (long)tile.start + threadIdx.x + u * NT;
       const bool valid = q < items, bil = q >= D.nc;
+      const long qf = (valid && !bil) ? q : 0;
+      ia[u] = gl(a.cA)[qf];
+      ib[u] = gl(a.cB)[qf];
 #pragma unroll
       for (int rr = 0; rr < 3; ++rr) {
         const bool on = valid && (rr == 0 || !bil);
         const long f = on ? (bil ? BROW(a, q - D.nc) : ROW(a, rr, q)) : 0;
         gk[u][rr] = Gk[f];
         rv[u][rr] = gl(a.r)[f];
-        nk[u][rr] = Nk[f];
         gm[u][rr] = Gkm1[f];  // read even after a restart (unused then)
         nm[u][rr] = Nkm1[f];
+        nk[u][rr] = (TM)0;
       }
-      const long qf = (valid && !bil) ? q : 0;
 #pragma unroll
       for (int k = 0; k < 6; ++k) fc[u][k] = k < NF ? gl(a.frame)[k * a.cap + qf] : (TM)0;
       pb[u] = gl(a.posB)[qf];
+    }
+    if (fresh) {
+      // rows that are not a contact gather body 0 (the id arrays hold at
+      // least one entry, but maybe no contact)
+      V4<TM> tA[R], tB[R];
+#pragma unroll
+      for (int u = 0; u < R; ++u) {
+        const long q = (long)tile.start + threadIdx.x + u * NT;
+        const bool own = q < D.nc;
+        tA[u] = ld4(a.tmp, own ? ia[u] : 0);
+        tB[u] = ld4(a.tmp, own ? ib[u] : 0);
+      }
+      DV_GLOBAL TM* Nw = gl(a.rn[0] + S.cur * a.ringStride);
+#pragma unroll
+      for (int u = 0; u < R; ++u) {
+        const long q = (long)tile.start + threadIdx.x + u * NT;
+        if (q >= items) continue;
+        if (q >= D.nc) {
+          nk[u][0] = bilRow(a, (int)(q - D.nc));
+          Nw[BROW(a, q - D.nc)] = nk[u][0];
+        } else {
+          TM F[9];
+          frameFrom(fc[u], D.geomMode, F);
+          rowsOfTmp(F, tA[u], tB[u], nk[u][0], nk[u][1], nk[u][2]);
+#pragma unroll
+          for (int rr = 0; rr < 3; ++rr) Nw[ROW(a, rr, q)] = nk[u][rr];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < R; ++u) {
+        const long q = (long)tile.start + threadIdx.x + u * NT;
+        const bool valid = q < items, bil = q >= D.nc;
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr) {
+          const bool on = valid && (rr == 0 || !bil);
+          nk[u][rr] = Nk[on ? (bil ? BROW(a, q - D.nc) : ROW(a, rr, q)) : 0];
+        }
+      }
     }
     const double bp = 1.0 + S.beta, bm = -S.beta;
     const double t = S.t;
@@ -1059,10 +1177,7 @@ __global__ __launch_bounds__(kTC / R, kP1MinWaves<R>) void kP1(const Args<TM>* _
           const long f = bil ? BROW(a, q - D.nc) : ROW(a, rr, q);
           Gn[f] = gn[rr];
           const double dy = gn[rr] - y[rr];
-          if (kMixed) {
-            gl(a.delta)[f] = (TM)dy;
-            acc[u][1] += dy * ny[rr];
-          } else {
+          if (!kMixed) {
             acc[u][0] += gn[rr] * (double)rv[u][rr];
             acc[u][1] += y[rr] * ny[rr];
             acc[u][2] += y[rr] * (double)rv[u][rr];
@@ -1100,105 +1215,28 @@ __global__ __launch_bounds__(kTC / R, kP1MinWaves<R>) void kP1(const Args<TM>* _
   }
 }
 
-template <typename TM, int R>
-__global__ __launch_bounds__(kTC / R) void kP3(const Args<TM>* __restrict__ A, Tables T, int par) {
-  constexpr bool kMixed = sizeof(TM) < sizeof(double);
-  constexpr int NT = kTC / R;
-  constexpr int NF = kMixed ? 6 : 3;
-  auto tileBody = [&](const Tile tile, const PassView& S, const Args<TM>& a) {
-    const SolveDims& D = a.dims;
-    const long items = (long)D.nc + D.nbil;
-    const DV_GLOBAL double* Gn = gl(a.ring[0] + S.nxt * a.ringStride);
-    DV_GLOBAL TM* Nn = gl(a.rn[0] + S.nxt * a.ringStride);
-    // loads of all contact rows of the thread, the body ids first: loads
-    // return in order, so the tmp gathers, which need the ids, wait only for
-    // them and go out while the rest are still in flight.  Every load is
-    // unconditional and every lane computes a contact row (rows not owned
-    // with contact 0's data and body 0), which only the owners store: a
-    // load under a row test would be a branch that waits for it.
-    double g[R][3];
-    TM dl[R][3], fc[R][6];
-    int ca[R], cb[R];
-#pragma unroll
-    for (int u = 0; u < R; ++u) {
-      const long q = (long)tile.start + threadIdx.x + u * NT;
-      const long qc = q < D.nc ? q : 0;  // contact 0 stands in for rows not owned
-      // the arrays hold at least one entry, but maybe no contact: rows not
-      // owned take body 0
-      const int ia = gl(a.cA)[qc], ib = gl(a.cB)[qc];
-      ca[u] = q < D.nc ? ia : 0;
-      cb[u] = q < D.nc ? ib : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < R; ++u) {
-      const long q = (long)tile.start + threadIdx.x + u * NT;
-      const long qc = q < D.nc ? q : 0;
-#pragma unroll
-      for (int rr = 0; rr < 3; ++rr) {
-        const long f = ROW(a, rr, qc);
-        g[u][rr] = kMixed ? 0.0 : Gn[f];  // the FP64 test needs gamma_new' N gamma_new
-        dl[u][rr] = kMixed ? gl(a.delta)[f] : (TM)0;
-      }
-#pragma unroll
-      for (int k = 0; k < 6; ++k) fc[u][k] = k < NF ? gl(a.frame)[k * a.cap + qc] : (TM)0;
-    }
-    V4<TM> tA[R], tB[R];
-#pragma unroll
-    for (int u = 0; u < R; ++u) {
-      tA[u] = ld4(a.tmp, ca[u]);
-      tB[u] = ld4(a.tmp, cb[u]);
-    }
-    // only slot 6 is used: one number per tile, packed so that the
-    // decision's loads of all tiles' partials touch few cache lines; one
-    // accumulator per row of the thread (see blockReduceR)
-    double a6[R][1];
-#pragma unroll
-    for (int u = 0; u < R; ++u) {
-      a6[u][0] = 0.0;
-      const long q = (long)tile.start + threadIdx.x + u * NT;
-      const bool own = q < D.nc;
-      TM F[9];
-      frameFrom(fc[u], D.geomMode, F);
-      const TM d0 = tA[u].x - tB[u].x, d1 = tA[u].y - tB[u].y, d2 = tA[u].z - tB[u].z;
-      const TM n0 = F[0] * d0 + F[1] * d1 + F[2] * d2;
-      const TM n1 = F[3] * d0 + F[4] * d1 + F[5] * d2;
-      const TM n2 = F[6] * d0 + F[7] * d1 + F[8] * d2;
-      const double c6 = kMixed ? (double)dl[u][0] * (double)n0 + (double)dl[u][1] * (double)n1 + (double)dl[u][2] * (double)n2
-                               : g[u][0] * (double)n0 + g[u][1] * (double)n1 + g[u][2] * (double)n2;
-      if (own) {
-        Nn[ROW(a, 0, q)] = n0; Nn[ROW(a, 1, q)] = n1; Nn[ROW(a, 2, q)] = n2;
-        a6[u][0] += c6;
-      } else if (q < items) {
-        const long f = BROW(a, q - D.nc);
-        const TM nv = bilRow(a, (int)(q - D.nc));
-        Nn[f] = nv;
-        a6[u][0] += (kMixed ? (double)gl(a.delta)[f] : Gn[f]) * (double)nv;
-      }
-    }
-    __shared__ double t6[1];
-    blockReduceR<NT, R, 1>(a6, t6);
-    if (threadIdx.x == 0) gl(a.q3[par])[tile.start / kTC] = t6[0];
-  };
-  if (T.single) {
-    // one slot: the grid covers the slot's tile capacity, and the tiles go
-    // to the workgroups in XCD-contiguous order (the tile count arrives
-    // with the table fields the loads need anyway); workgroups past the
-    // count have no tile.  There is no test of the view's done flag, so that
-    // no branch waits on the view before the loads.  After the solve has
-    // ended the pass's stores are simply not used: N of ring slot nxt is
-    // never gamma-hat's, and the final kernels rebuild what they read.
-    const PassView S = loadView(T, 0, par ^ 1);
-    const Args<TM>& a = A[0];
-    const TileRange tr = tilesOf(T, a.nTiles, gridDim.x);
-    for (int ti = tr.first; ti < tr.end; ti += tr.step) tileBody(Tile{0, ti * kTC}, S, a);
+// (N x)_c = F_c (tmp_A - tmp_B) and the bilateral rows of x whose
+// M^-1 D^T x is in tmp, into N of ring slot nxt (the view of parity par ^ 1),
+// one row tile per workgroup of a single slot: the rows a pass's P1 forms
+// for a fresh iterate, as a kernel of their own for dv_pass_product.
+template <typename TM>
+__global__ __launch_bounds__(kTC) void kRowsN(const Args<TM>* __restrict__ A, Tables T, int par) {
+  const PassView S = loadView(T, 0, par ^ 1);
+  const Args<TM>& a = A[0];
+  const SolveDims& D = a.dims;
+  const long items = (long)D.nc + D.nbil;
+  const long q = (long)blockIdx.x * kTC + threadIdx.x;
+  if (q >= items) return;
+  DV_GLOBAL TM* Nn = gl(a.rn[0] + S.nxt * a.ringStride);
+  if (q >= D.nc) {
+    Nn[BROW(a, q - D.nc)] = bilRow(a, (int)(q - D.nc));
     return;
   }
-  TILE_LOOP(T) {
-    const Tile tile = rowTile(T, ti_);
-    const PassView S = loadView(T, tile.scene, par ^ 1);
-    if (viewDone(S)) continue;
-    tileBody(tile, S, A[tile.scene]);
-  }
+  TM F[9];
+  ldFrame(a, (int)q, D.geomMode, F);
+  TM n0, n1, n2;
+  rowsOfTmp(F, ld4(a.tmp, gl(a.cA)[q]), ld4(a.tmp, gl(a.cB)[q]), n0, n1, n2);
+  Nn[ROW(a, 0, q)] = n0; Nn[ROW(a, 1, q)] = n1; Nn[ROW(a, 2, q)] = n2;
 }
 
 // Decision of the pass of parity par for every slot (one wavefront each):
@@ -1214,7 +1252,7 @@ __global__ __launch_bounds__(kTP) void kDecide(const Args<TM>* __restrict__ A, T
     const Args<TM>& a = A[0];
     ApgdState S = STATE(T, 0, par ^ 1);
     double tot[2 * kSlots];
-    passTotals(a.p1sum[par], a.q3[par], a.nTiles, tot);
+    passTotals(a.p1sum[par], a.lq[par], a.nBodyTiles + gl(a.counters)[2], tot);
     if (!S.done) decideStep<kMixed>(S, tot, a.dims);
     if (threadIdx.x == 0) {
       STATE(T, 0, par) = S;
@@ -1228,7 +1266,7 @@ __global__ __launch_bounds__(kTP) void kDecide(const Args<TM>* __restrict__ A, T
   ApgdState S = STATE(T, blockIdx.x, par ^ 1);
   if (!S.done) {
     double tot[2 * kSlots];
-    reducePass(a.part1[par], a.q3[par], a.nTiles, tot);
+    reducePass(a.part1[par], a.nTiles, a.lq[par], a.nBodyTiles + gl(a.counters)[2], tot);
     decideStep<kMixed>(S, tot, a.dims);
   }
   if (threadIdx.x == 0) {
@@ -1322,7 +1360,7 @@ __global__ __launch_bounds__(kTD) void kFinDecide(const Args<TM>* __restrict__ A
 // sum
 template <typename TM>
 __global__ __launch_bounds__(1024) void kHeavyTable(const Args<TM>* __restrict__ A, const int* slots, int nSlots,
-                                                     int2* refs, int* counts) {
+                                                     int2* refs, int* refIdx, int* counts) {
   __shared__ int sc[1024];
   int base = 0;
   for (int s0 = 0; s0 < nSlots; s0 += 1024) {
@@ -1338,7 +1376,10 @@ __global__ __launch_bounds__(1024) void kHeavyTable(const Args<TM>* __restrict__
       __syncthreads();
     }
     const int off = base + sc[threadIdx.x] - n;
-    for (int h = 0; h < n; ++h) refs[off + h] = make_int2(s, A[s].heavy[h]);
+    for (int h = 0; h < n; ++h) {
+      refs[off + h] = make_int2(s, A[s].heavy[h]);
+      refIdx[off + h] = h;  // its place in the slot's list, where P2 puts its Lipschitz term
+    }
     base += sc[1023];
     __syncthreads();
   }
@@ -1362,11 +1403,7 @@ void launchP1(const Plan<TM>& P, hipStream_t st, int par, bool fold) {
   else if (fold) hipLaunchKernelGGL((kP1<TM, 1, true>), dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
   else hipLaunchKernelGGL((kP1<TM, 1, false>), dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
 }
-template <typename TM>
-void launchP3(const Plan<TM>& P, hipStream_t st, int par) {
-  if (P.rpt == 2) hipLaunchKernelGGL((kP3<TM, 2>), dim3(P.gc), dim3(kTC / 2), 0, st, P.A, P.T, par);
-  else hipLaunchKernelGGL((kP3<TM, 1>), dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
-}
+
 
 constexpr int kProfBurst = 16;  // timed back-to-back launches of the profiled kernel
 
@@ -1376,11 +1413,10 @@ void launchPass(const Plan<TM>& P, hipStream_t st, int kern, int par, bool fold)
   // a single slot's P2 has one more workgroup, which totals P1's partials
   if (kern == 2)
     hipLaunchKernelGGL((kBody<TM, 2>), dim3(P.gb + (P.T.single ? 1 : 0)), dim3(kTB), 0, st, P.A, P.T, par ^ 1);
-  if (kern == 3) hipLaunchKernelGGL((kP3<TM, 1>), dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, par);
   if (kern == 4) hipLaunchKernelGGL(kDecide<TM>, dim3(P.nSlots), dim3(kTP), 0, st, P.A, P.T, par);
 }
 
-// kProfBurst launches of pass kernel `kern` (P1, P2 or P3) between ev[0] and
+// kProfBurst launches of pass kernel `kern` (P1 or P2) between ev[0] and
 // ev[1], as a captured graph like the passes themselves, so that the average
 // is the kernel's duration in a graph rather than that plus a stream
 // launch's latency.  The graph is kept for the next solve of the same plan.
@@ -1416,8 +1452,9 @@ void timedBurst(const Plan<TM>& P, hipStream_t st, int kern, int par, bool fold,
 // kernel once, after the last pass (the host reads the states after a
 // chunk), and the first pass's P1 reads the view that kernel left.  With `ev`
 // (stream launches only: events recorded inside a captured graph cannot be
-// timed), kernel `prof` runs as a timed burst (P1, P2 and P3 read nothing
-// they write, so the repeats reproduce the same outputs) in the chunk's first
+// timed), kernel `prof` runs as a timed burst (P1 and P2 read nothing they
+// write, apart from the N that a P1 forms for a fresh iterate and every
+// repeat forms the same, so the repeats reproduce the same outputs) in the chunk's first
 // pass, or for a P1 that decides, its second (whose decision has this
 // chunk's partial sums); the decide kernel, which is not repeatable, is
 // timed alone where it runs.
@@ -1428,6 +1465,7 @@ void enqueuePasses(const Plan<TM>& P, hipStream_t st, int n, hipEvent_t* ev, int
     const int par = k & 1;
     const bool fold = single && k > 0;
     for (int kern = 1; kern <= 4; ++kern) {
+      if (kern == 3) continue;  // (a pass's contact rows are formed inside the next P1)
       if (kern == 4 && single && k != n - 1) continue;
       const int kt = !single ? 0 : kern == 1 ? 1 : kern == 4 ? n - 1 : 0;
       const bool timed = ev && prof == kern && k == kt;
@@ -1632,6 +1670,7 @@ void TileSet::release() {
   items.release();
   bodies.release();
   heavy.release();
+  heavyIdx.release();
   counts.release();
   slots.release();
   hItems.clear();
@@ -1676,13 +1715,17 @@ void prepareScene(Scene& s, Args<TM>& a) {
   const long RsCap = std::max<long>(3L * s.ncCap + s.nbil, 1);
   const long itemsCap = (long)s.ncCap + s.nbil;
   const long tilesCap = std::max<long>(1, (itemsCap + kTC - 1) / kTC);
-  s.partials.reserve((size_t)4 * kSlots * tilesCap + 2 * kSlots + 2 * tilesCap);
+  const long bodyTiles = std::max<long>(1, ((long)s.nb + kTB - 1) / kTB);
+  // a body reduced by a workgroup has more than kHeavyDegree of the 2 nc incidences
+  const long lqCap = bodyTiles + 2L * std::max(s.ncCap, 1) / kHeavyDegree + 2;
+  s.partials.reserve((size_t)4 * kSlots * tilesCap + 2 * kSlots + 2 * lqCap);
   s.work.reserve((size_t)5 * RsCap + 16);  // ring[4], gHat: FP64 in both solvers
-  // TM block: rn[4], r, delta, then J and JB (per contact) and tmp (per
-  // body) of 4 components 32-byte aligned, then (mixed) M^-1
-  const size_t tmElems = (size_t)6 * RsCap + 8 +
-                         4 * (2 * (size_t)std::max(s.ncCap, 1) + (size_t)std::max(s.nb, 1)) +
-                         (kMixed ? (size_t)s.nb + 1 : 0) + 16;
+  // TM block: rn[4], r, then J and JB (per contact), tmp and (mixed) the
+  // four ring slots of D^T gamma (per body) of 4 components 32-byte aligned,
+  // then (mixed) M^-1
+  const size_t nbc = (size_t)std::max(s.nb, 1);
+  const size_t tmElems = (size_t)5 * RsCap + 8 + 4 * (2 * (size_t)std::max(s.ncCap, 1) + nbc + (kMixed ? 4 * nbc : 0)) +
+                         (kMixed ? nbc + 1 : 0) + 16;
   TM* WM;
   if (kMixed) {
     s.frame32.reserve((size_t)6 * std::max(s.ncCap, 1));  // n, u
@@ -1700,6 +1743,7 @@ void prepareScene(Scene& s, Args<TM>& a) {
   a.dims.warm = s.warm ? 1 : 0; a.dims.geomMode = s.geomMode;
   const long items = (long)s.nc + s.nbil;
   a.nTiles = (int)((items + kTC - 1) / kTC);
+  a.nBodyTiles = (int)((s.nb + kTB - 1) / kTB);
   a.active = 1;
   a.frame64 = s.frame.p;
   a.cap = s.ncCap;
@@ -1715,15 +1759,15 @@ void prepareScene(Scene& s, Args<TM>& a) {
   a.gHat = W + 4 * RsCap;
   for (int k = 0; k < 4; ++k) a.rn[k] = WM + k * RsCap;
   a.r = WM + 4 * RsCap;
-  a.delta = kMixed ? WM + 5 * RsCap : nullptr;
-  TM* after = WM + 6 * RsCap;
+  TM* after = WM + 5 * RsCap;
   uintptr_t p = (reinterpret_cast<uintptr_t>(after) + 31) & ~uintptr_t(31);
   a.J = reinterpret_cast<V4<TM>*>(p);
   a.JB = a.J + std::max<long>(s.ncCap, 1);
   a.tmp = a.JB + std::max<long>(s.ncCap, 1);
   if (kMixed) {
+    a.sr = a.tmp + nbc;
     a.frame = reinterpret_cast<TM*>(s.frame32.p);
-    a.minv = reinterpret_cast<TM*>(a.tmp + std::max(s.nb, 1));
+    a.minv = reinterpret_cast<TM*>(a.sr + 4 * nbc);
   } else {
     a.frame = reinterpret_cast<TM*>(s.frame.p);
     a.minv = reinterpret_cast<TM*>(s.minv.p);
@@ -1732,7 +1776,7 @@ void prepareScene(Scene& s, Args<TM>& a) {
     a.part1[k] = s.partials.p + (long)(2 * k) * kSlots * tilesCap;
     a.part3[k] = s.partials.p + (long)(2 * k + 1) * kSlots * tilesCap;
     a.p1sum[k] = s.partials.p + 4L * kSlots * tilesCap + k * kSlots;
-    a.q3[k] = s.partials.p + 4L * kSlots * tilesCap + 2 * kSlots + k * tilesCap;
+    a.lq[k] = s.partials.p + 4L * kSlots * tilesCap + 2 * kSlots + k * lqCap;
   }
 }
 
@@ -1757,6 +1801,7 @@ void buildSet(Engine& E, TileSet& set, hipStream_t st) {
   set.items.reserve(std::max<size_t>(set.hItems.size(), 1));
   set.bodies.reserve(std::max<size_t>(set.hBodies.size(), 1));
   set.heavy.reserve((size_t)heavyCap);
+  set.heavyIdx.reserve((size_t)heavyCap);
   set.counts.reserve(4);
   set.slots.reserve(std::max<size_t>(set.hSlots.size(), 1));
   set.countsHost[0] = (int)set.hItems.size();
@@ -1775,7 +1820,7 @@ void buildSet(Engine& E, TileSet& set, hipStream_t st) {
   if (!set.hSlots.empty())
     hipLaunchKernelGGL(kHeavyTable<TM>, dim3(1), dim3(1024), 0, st,
                        reinterpret_cast<const Args<TM>*>(E.table.p), set.slots.p, (int)set.hSlots.size(),
-                       set.heavy.p, set.counts.p);
+                       set.heavy.p, set.heavyIdx.p, set.counts.p);
 }
 
 template <typename TM>
@@ -1787,6 +1832,7 @@ Tables tablesOf(Engine& E, TileSet& set) {
   T.items = set.items.p;
   T.bodies = set.bodies.p;
   T.heavy = set.heavy.p;
+  T.heavyIdx = set.heavyIdx.p;
   T.counts = set.counts.p;
   T.slots = set.slots.p;
   T.out = reinterpret_cast<ApgdState*>(E.out.p);
@@ -2135,10 +2181,10 @@ namespace {
 // N x by the kernels a solver pass runs, on the assembled scene alone: the
 // setup takes gamma_0 = x (warm start from the staging copy of x) and stores
 // J = F^T x with the store P1 uses; then P2 (kBody<TM, 2>) forms tmp = M^-1
-// (sum_A J - sum_B J + bilateral rows of x) and P3 (kP3) the rows
-// (N x)_c = F_c (tmp_A - tmp_B) and the bilateral rows, into N of ring slot 1
-// (the pass's gamma_new slot, which gets a copy of x).  y receives them in
-// the reference's flat layout, as TM values.
+// (sum_A J - sum_B J + bilateral rows of x) and kRowsN the rows
+// (N x)_c = F_c (tmp_A - tmp_B) and the bilateral rows, as the next P1 forms
+// them, into N of ring slot 1 (the pass's gamma_new slot, which gets a copy
+// of x).  y receives them in the reference's flat layout, as TM values.
 template <typename TM>
 void passProductT(Scene& s, const double* xDev, double* yHost) {
   Engine& E = s.engine;
@@ -2153,7 +2199,7 @@ void passProductT(Scene& s, const double* xDev, double* yHost) {
   DV_HIP(hipMemcpyAsync(a.ring[1], a.ring[0], sizeof(double) * a.ringStride, hipMemcpyDeviceToDevice, st));
   const Plan<TM> P = activePlan<TM>(E);
   launchPass<TM>(P, st, 2, 0, false);  // kBody<TM, 2> on the view the setup left (nxt = 1)
-  launchP3<TM>(P, st, 0);
+  hipLaunchKernelGGL(kRowsN<TM>, dim3(P.gc), dim3(kTC), 0, st, P.A, P.T, 0);
   std::vector<TM> rn((size_t)a.ringStride);
   DV_HIP(hipMemcpyAsync(rn.data(), a.rn[1], sizeof(TM) * rn.size(), hipMemcpyDeviceToHost, st));
   DV_HIP(hipStreamSynchronize(st));

@@ -291,7 +291,7 @@ int dv_scene_destroy(dv_scene* h) {
   dv::DevBuf<int>* ibufs[] = {&s.family, &s.smallIds, &s.bigIds, &s.bilA, &s.bilB, &s.bilIncStart,
                               &s.bilInc, &s.cellItems, &s.cntUp, &s.cntDown, &s.cStart, &s.iStart,
                               &s.scanTmp, &s.counters, &s.hostTotals, &s.cA, &s.cB, &s.inc, &s.posB,
-                              &s.heavy, &s.bigChunks};
+                              &s.heavy, &s.heavyPos, &s.bigChunks};
   for (auto* b : ibufs) b->release();
   s.slotOf.release(); s.slotCount.release(); s.slotStart.release(); s.slotFill.release();
   s.frame32.release(); s.work32.release();
@@ -393,7 +393,7 @@ int dv_set_param(dv_scene* h, int param, double v) {
       case DV_PARAM_VERBOSE: s.verbose = (int)v; break;
       case DV_PARAM_GRID: s.grid = std::max(0, (int)v); break;
       case DV_PARAM_PROFILE:
-        if (v < 0 || v > 4) return dv::fail(s, DV_ERR_ARG, "profile kernel is 0..4");
+        if (v < 0 || v > 4 || v == 3) return dv::fail(s, DV_ERR_ARG, "profile kernel is 0, 1, 2 or 4");
         s.profKernel = (int)v;
         break;
       case DV_PARAM_ROWS_PER_THREAD:
@@ -835,12 +835,14 @@ void needScene(dv_scene* h) {
 
 // Device memory the solver of a slot needs beyond the scene's own state:
 // the FP64 iterate ring (4 slots + gammaHat), the product vectors (4 N gamma,
-// r, delta), J and JB, tmp, the FP32 frames and the partial sums.
+// r), J and JB, tmp, (mixed) the ring of D^T gamma, M^-1 and the FP32 frames,
+// and the partial sums (per contact tile, per body tile and per heavy body).
 double solverBytes(const Scene& s, bool mixed) {
   const double rows = 3.0 * std::max(s.ncCap, 1) + s.nbil;
+  const double nc = std::max(s.ncCap, 1), nb = std::max(s.nb, 1);
   const double tm = mixed ? 4 : 8;
-  return rows * (5 * 8 + 6 * tm) + std::max(s.ncCap, 1) * (8 * tm + (mixed ? 24 : 0)) +
-         std::max(s.nb, 1) * (5 * tm) + 4.0 * 8 * 8 * (1 + (rows / 3 + 1023) / 1024);
+  return rows * (5 * 8 + 5 * tm) + nc * (8 * tm + (mixed ? 24 : 0)) + nb * ((mixed ? 21 : 4) * tm) +
+         4.0 * 8 * 8 * (1 + (rows / 3 + 1023) / 1024) + 2 * 8 * (nb / 256 + 2 * nc / kHeavyDegree + 2) + 256;
 }
 
 // f(i) for i in [0, n) on up to eight host threads (each scene has its own

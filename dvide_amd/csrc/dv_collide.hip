@@ -302,8 +302,7 @@ __global__ __launch_bounds__(kT) void emitBigChunks(CdArgs a, EmitOut o, int nch
 // every body are contiguous there
 __global__ __launch_bounds__(kT) void buildIncidence(int nb, const int* cStart, const int* iStart,
                                                      const int* cntUp, const int* cntDown,
-                                                     const int* cB, int* inc, int* posB, int* heavy,
-                                                     int* counters) {
+                                                     const int* cB, int* inc, int* posB, int* heavyFlag) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nb) return;
   const int i0 = iStart[g], nd = cntDown[g], nu = cntUp[g];
@@ -320,7 +319,16 @@ __global__ __launch_bounds__(kT) void buildIncidence(int nb, const int* cStart, 
   }
   const int c0 = cStart[g];
   for (int q = 0; q < nu; ++q) inc[i0 + nd + q] = (c0 + q) << 1;
-  if (nd + nu > kHeavyDegree) heavy[atomicAdd(&counters[2], 1)] = g;
+  heavyFlag[g] = nd + nu > kHeavyDegree ? 1 : 0;
+}
+
+// the bodies reduced by a whole workgroup, in ascending order (so that
+// everything summed over them is summed in a fixed order), and their count
+__global__ __launch_bounds__(kT) void compactHeavy(int nb, const int* heavyFlag, const int* pos, int* heavy,
+                                                   int* counters) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g == 0) counters[2] = pos[nb];
+  if (g < nb && heavyFlag[g]) heavy[pos[g]] = g;
 }
 
 __global__ void zeroInt(int* x, int n) {
@@ -376,6 +384,7 @@ void collideInit(Scene& s) {
   s.iStart.reserve(s.nb + 1);
   s.counters.reserve(8);
   s.heavy.reserve(s.nb + 1);
+  s.heavyPos.reserve(s.nb + 1);
   DV_HIP(hipStreamSynchronize(s.stream));
 }
 
@@ -470,8 +479,12 @@ void collideEmit(Scene& s) {
   if (s.nsmall) hipLaunchKernelGGL(emitSmall, dim3(blocksFor(s.nsmall, kT)), dim3(kT), 0, st, a, o);
   if (s.nbig)
     hipLaunchKernelGGL(emitBigChunks, dim3(nchunks, s.nbig), dim3(kT), 0, st, a, o, nchunks, chUp, chDown);
+  int* heavyFlag = s.hostTotals.p;  // free again after the offsets' scans
   hipLaunchKernelGGL(buildIncidence, dim3(blocksFor(s.nb, kT)), dim3(kT), 0, st, s.nb, s.cStart.p,
-                     s.iStart.p, s.cntUp.p, s.cntDown.p, s.cB.p, s.inc.p, s.posB.p, s.heavy.p, s.counters.p);
+                     s.iStart.p, s.cntUp.p, s.cntDown.p, s.cB.p, s.inc.p, s.posB.p, heavyFlag);
+  exclusiveScan(s, heavyFlag, s.heavyPos.p, s.nb);
+  hipLaunchKernelGGL(compactHeavy, dim3(blocksFor(s.nb, kT)), dim3(kT), 0, st, s.nb, (const int*)heavyFlag,
+                     (const int*)s.heavyPos.p, s.heavy.p, s.counters.p);
   DV_HIP(hipGetLastError());
 }
 

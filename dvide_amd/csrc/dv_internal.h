@@ -101,6 +101,7 @@ struct Tile {
 struct TileSet {
   DevBuf<Tile> items, bodies;
   DevBuf<int2> heavy;
+  DevBuf<int> heavyIdx;  // each heavy body's index in its slot's heavy list
   DevBuf<int> counts;  // [0] row tiles, [1] body tiles, [2] heavy bodies
   DevBuf<int> slots;
   std::vector<Tile> hItems, hBodies;
@@ -114,7 +115,7 @@ struct TileSet {
 // keep the one-row-per-thread order), and the tile order and grid sizes only
 // decide which workgroup takes which tile.
 struct Schedule {
-  int rpt = 1;  // contact rows per thread of P1 and P3 (1 or 2)
+  int rpt = 1;  // contact rows per thread of P1 (1 or 2)
   int xcd = 1;  // XCD-contiguous tile order for the gathering kernels
   int wgc = 32; // several slots: contact-kernel workgroups per CU
   int wgb = 32; // several slots: body-kernel workgroups per CU (one slot: 4)
@@ -212,7 +213,8 @@ struct Scene {
   DevBuf<float> frame32; // FP32 copy for the mixed solver
   DevBuf<int> inc;       // per-body incidences (contact << 1 | side)
   DevBuf<int> posB;      // per contact: its slot in the B-side order (see buildIncidence)
-  DevBuf<int> heavy;     // bodies processed by a whole workgroup
+  DevBuf<int> heavy;     // bodies processed by a whole workgroup, ascending
+  DevBuf<int> heavyPos;  // their positions in `heavy` (exclusive scan of the flags)
   DevBuf<int> bigChunks; // per (big geometry, chunk) partner counts
 
   // --- system of the step (flat rows: nbil bilateral, then 3 per contact)
@@ -253,7 +255,7 @@ void minvTimes(Scene& s, const double* x, double* y);  // y = M^-1 x per DOF
 // solvers (dv_apgd_plain.hip, dv_apgd_fused.hip)
 void solvePlain(Scene& s);
 void solveFused(Scene& s, bool mixed);
-// N x by the graph-driven solver's own pass kernels (J store, P2, P3) on the
+// N x by the graph-driven solver's own pass kernels (J store, P2, contact rows) on the
 // assembled scene; x is a flat device array, y a flat host array
 void passProduct(Scene& s, bool mixed, const double* xDev, double* yHost);
 // one solve of several scenes in the same launches (on stream st); each

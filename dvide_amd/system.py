@@ -26,7 +26,7 @@ BUILD_FP64, BUILD_MIXED, BUILD_PLAIN = 0, 1, 2  # DV_APGD_*
 (PARAM_H, PARAM_MU, PARAM_TOL, PARAM_MAX_ITER, PARAM_GX, PARAM_GY, PARAM_GZ, PARAM_ENVELOPE,
  PARAM_WARM_START, PARAM_APGD_BUILD, PARAM_GEOM_MODE, PARAM_TIME, PARAM_VERBOSE, PARAM_GRID,
  PARAM_PROFILE, PARAM_ROWS_PER_THREAD, PARAM_ANTI_RELAX) = range(17)
-PROFILE_OFF, PROFILE_P1, PROFILE_P2, PROFILE_P3, PROFILE_DECIDE = range(5)
+PROFILE_OFF, PROFILE_P1, PROFILE_P2, PROFILE_DECIDE = 0, 1, 2, 4  # DV_PARAM_PROFILE values
 VEC_P, VEC_V, VEC_GAMMA, VEC_F_CONTACT, VEC_R, VEC_K, VEC_B = range(7)
 
 EXPORTED = (

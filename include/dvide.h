@@ -74,7 +74,8 @@ typedef struct dv_scene dv_scene;
 #define DV_PARAM_PROFILE 14   /* time one kernel of the graph-driven solvers with HIP
                                  events: 0 off, 1 P1 (contacts; for one scene it also
                                  takes the previous pass's decision), 2 P2 (bodies),
-                                 3 P3 (contacts), 4 the decide kernel.  One chunk of 16
+                                 4 the decide kernel (3 is refused: a pass has two
+                                 kernels, P1 and P2).  One chunk of 16
                                  passes per solve (the first after the first chunk whose
                                  entry state has the momentum term on) is then launched
                                  on the scene's stream, and in its first pass (a deciding

@@ -54,8 +54,8 @@ def test_full_size_contacts_assembly_and_product(gpu, workload):
 
 @pytest.mark.parametrize("solver,bound", [(system.BUILD_FP64, 1e-13), (system.BUILD_MIXED, 2e-6)])
 def test_full_size_pass_product(gpu, workload, solver, bound):
-    """The product the solvers iterate with (J = F^T x, P2's gather, P3's
-    rows) against the oracle's N x at the full size: FP64 to rounding, the
+    """The product the solvers iterate with (J = F^T x, P2's gather, the
+    contact rows the next P1 forms) against the oracle's N x at the full size: FP64 to rounding, the
     mixed build to binary32 rounding."""
     o, g = both(workload, tol=TOL)
     nc = o.assemble()

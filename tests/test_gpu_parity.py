@@ -192,7 +192,7 @@ def test_free_fall_gpu(gpu):
 
 
 def test_rows_per_thread_agree(gpu):
-    """One or two contacts per thread in the contact passes P1 and P3 only
+    """One or two contacts per thread in the contact pass P1 only
     move rows between lanes: the partial sums keep the one-row-per-thread
     order, so the same ten iterations are bit-identical (both builds)."""
     sc = scenes.pile(31, 6000)
@@ -206,6 +206,31 @@ def test_rows_per_thread_agree(gpu):
             g.close()
         assert out[0][0] == out[1][0] == 10
         assert bits_equal(out[1][1], out[0][1]) and bits_equal(out[1][2], out[0][2]), solver
+
+
+def test_profiled_solves_are_unchanged(gpu):
+    """Timing P1, P2 or the decide kernel (DV_PARAM_PROFILE) repeats a
+    kernel in a burst, which must reproduce its outputs: the same 60
+    iterations are bit-identical with and without it (both builds).  A pass
+    has two kernels, so kernel 3 is refused."""
+    sc = scenes.pile(31, 6000)
+    for solver in (system.BUILD_FP64, system.BUILD_MIXED):
+        out = {}
+        for k in (system.PROFILE_OFF, system.PROFILE_P1, system.PROFILE_P2, system.PROFILE_DECIDE):
+            g = system.from_scene(sc, solver, tol=0.0, max_iterations=60)
+            g.setProfileKernel(k)
+            g.step()
+            st = g.stats()
+            assert st.iterations == 60
+            assert (st.prof_kernel_count > 0) == (k != system.PROFILE_OFF), (solver, k)
+            out[k] = (g.gamma.copy(), g.v.copy())
+            g.close()
+        for k in (system.PROFILE_P1, system.PROFILE_P2, system.PROFILE_DECIDE):
+            assert bits_equal(out[k][0], out[0][0]) and bits_equal(out[k][1], out[0][1]), (solver, k)
+    g = system.from_scene(sc, system.BUILD_MIXED)
+    with pytest.raises(system.DvideError):
+        g.setProfileKernel(3)
+    g.close()
 
 
 def test_fp64_solver_velocities_match_pgs(gpu):
@@ -232,7 +257,7 @@ def test_fp64_solver_velocities_match_pgs(gpu):
 @pytest.mark.parametrize("solver,bound", [(system.BUILD_FP64, 1e-13), (system.BUILD_MIXED, 2e-6)])
 def test_pass_product_matches_oracle(gpu, solver, bound):
     """The product the graph-driven solvers iterate with -- J = F^T x per
-    contact, P2's per-body gather and P3's contact rows -- against the
+    contact, P2's per-body gather and the contact rows a P1 forms from it -- against the
     oracle's COO product D (M^-1 (D^T x)) on a random x, bilateral rows
     included: FP64 to rounding, the mixed build to binary32 rounding (the
     frames, M^-1, J and tmp are binary32 there)."""
