@@ -353,15 +353,16 @@ __device__ void reduceAll(const double* part1, const double* part3, int ng, doub
 // Fixed-order totals over a slot's per-tile partials of NS consecutive slots
 // from s0 (the last slot, kSlots - 1, is a max, the others sums), by a
 // workgroup of kTP threads: thread t adds tiles t, t + kTP, ... in order (the
-// loads of eight tiles issued together), each wavefront combines its lanes by
+// loads of K tiles issued together; K sets only how many are in flight, not
+// the order), each wavefront combines its lanes by
 // a butterfly, and the wavefront totals are added in wavefront order.  out[]
 // holds the NS totals in every thread.  The pass decision reads P1's slots
 // and P2's Lipschitz partials this way, wherever it takes them from.  In a wider
 // workgroup the threads past kTP add nothing, so the totals are the same.
 constexpr int kTP = 256;
-template <int NS, int kStride = kSlots>
+template <int NS, int kStride = kSlots, int K = 8>
 __device__ void reduceSlots(const double* part, int s0, int ng, double (&out)[NS]) {
-  constexpr int K = 8, W = kTP / 64;
+  constexpr int W = kTP / 64;
   __shared__ double sh[W][NS];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool mine = threadIdx.x < kTP;
@@ -662,7 +663,7 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
     const Args<TM>& a = A[0];
     const int par = sIdx ^ 1;
     double t1[kSlots];
-    reduceSlots<kSlots>(a.part1[par], 0, a.nTiles, t1);
+    reduceSlots<kSlots, kSlots, 2>(a.part1[par], 0, a.nTiles, t1);  // (two tiles' loads at a time: the same order)
     if (threadIdx.x < kSlots) gl(a.p1sum[par])[threadIdx.x] = t1[threadIdx.x];
     return;
   }
@@ -1042,7 +1043,19 @@ __global__ __launch_bounds__(kTC / R, (kP1MinWaves<TM, R>)) void kP1(const Args<
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   constexpr int NT = kTC / R;
   constexpr int NF = kMixed ? 6 : 3;  // stored frame components
-  auto tileBody = [&](const Tile tile, const PassView& S, const Args<TM>& a) {
+  // the contacts' body ids, which the rows of N gamma_k gather with: a single
+  // slot loads them before it takes the previous decision, so that the
+  // gathers of a fresh iterate's rows go out with the iterate's loads
+  auto loadIds = [&](const Tile tile, const Args<TM>& a, int (&ia)[R], int (&ib)[R]) {
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const long q = (long)tile.start + threadIdx.x + u * NT;
+      const long qf = q < a.dims.nc ? q : 0;
+      ia[u] = gl(a.cA)[qf];
+      ib[u] = gl(a.cB)[qf];
+    }
+  };
+  auto tileBody = [&](const Tile tile, const PassView& S, const Args<TM>& a, const int (&ia)[R], const int (&ib)[R]) {
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
     const DV_GLOBAL double* Gk = gl(a.ring[0] + S.cur * a.ringStride);
@@ -1062,14 +1075,11 @@ __global__ __launch_bounds__(kTC / R, (kP1MinWaves<TM, R>)) void kP1(const Args<
     double gk[R][3], gm[R][3];
     TM rv[R][3], nk[R][3], nm[R][3], fc[R][6];
     int pb[R];  // the contacts' B-side slots, loaded with the rest
-    int ia[R], ib[R];
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       const long q = (long)tile.start + threadIdx.x + u * NT;
       const bool valid = q < items, bil = q >= D.nc;
       const long qf = (valid && !bil) ? q : 0;
-      ia[u] = gl(a.cA)[qf];
-      ib[u] = gl(a.cB)[qf];
 #pragma unroll
       for (int rr = 0; rr < 3; ++rr) {
         const bool on = valid && (rr == 0 || !bil);
@@ -1202,16 +1212,21 @@ __global__ __launch_bounds__(kTC / R, (kP1MinWaves<TM, R>)) void kP1(const Args<
     // only their partial sums, which no decision reads, so no branch waits
     // on the tile count; workgroup 0 stores the decided state.
     const Args<TM>& a = A[0];
+    const Tile tile{0, (int)blockIdx.x * kTC};
+    int ia[R], ib[R];
+    loadIds(tile, a, ia, ib);
     const PassView S = kFold ? decidePrevious(a, T, par) : loadView(T, 0, par ^ 1);
     if (viewDone(S)) return;
-    tileBody(Tile{0, (int)blockIdx.x * kTC}, S, a);
+    tileBody(tile, S, a, ia, ib);
     return;
   }
   TILE_LOOP(T) {
     const Tile tile = rowTile(T, ti_);
     const PassView S = loadView(T, tile.scene, par ^ 1);
     if (viewDone(S)) continue;
-    tileBody(tile, S, A[tile.scene]);
+    int ia[R], ib[R];
+    loadIds(tile, A[tile.scene], ia, ib);
+    tileBody(tile, S, A[tile.scene], ia, ib);
   }
 }
 
