@@ -180,6 +180,49 @@ def test_server_rejects_a_second_request_in_flight(gpu):
     srv.close()
 
 
+def test_server_diverged_request_frees_its_slot(gpu):
+    """A request whose solve turns non-finite (a sphere centre inside a box
+    has no normal, so D has NaN rows) fails with DV_ERR_DIVERGED after its
+    slot is freed: its ticket is reported by failed(), the server drains,
+    and with two slots the third request, which waits for a free one, is
+    still admitted; the others are solved as they are alone."""
+    solver = system.BUILD_MIXED
+    bad = system.from_scene(scenes.cloud(31, n=400, edge=0.6), solver)
+
+    def good():
+        return [system.from_scene(make("pile", 41, 700), solver, tol=1e-5, max_iterations=3000),
+                system.from_scene(make("pile", 42, 500), solver, tol=1e-5, max_iterations=3000)]
+
+    alone, served = good(), good()
+    for g in alone:
+        g.step()
+    srv = system.Server(solver, max_slots=2)
+    t_bad = srv.submit(bad)
+    t_good = [srv.submit(g) for g in served]
+    done, errors = [], 0
+    for _ in range(100000):  # a chunk per poll; bounded so that a stuck server fails the test
+        if not srv.pending():
+            break
+        try:
+            done += srv.poll()
+        except system.DvideError as e:
+            assert e.code == -6  # DV_ERR_DIVERGED
+            errors += 1
+            done += e.done
+    assert srv.pending() == 0
+    assert errors == 1 and srv.failed() == [t_bad]
+    assert sorted(done) == sorted(t_good)
+    for i, (a, b) in enumerate(zip(alone, served)):
+        same_state(a, b, ("after the diverged request", i))
+    for g in alone:  # the freed slots take the next requests
+        g.step()
+    again = [srv.submit(g) for g in served]
+    assert sorted(srv.drain()) == sorted(again)
+    for i, (a, b) in enumerate(zip(alone, served)):
+        same_state(a, b, ("second requests", i))
+    srv.close()
+
+
 def test_batch_rejects_the_plain_solver(gpu):
     with pytest.raises(system.DvideError):
         system.Batch(system.BUILD_PLAIN)
