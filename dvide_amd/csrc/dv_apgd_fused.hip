@@ -183,7 +183,7 @@ struct Args {
   V4<TM>* sr;           // mixed solver: D^T of each ring slot's iterate, per body (slot k at k * nb)
   double* gHat;         // gammaHat before any iterate is accepted (ones)
   double* part1[2];     // kSlots per tile, by pass parity
-  double* part3[2];
+  double* part3[2];     // setup and final kernels: a second kSlots per tile
   double* lq[2];        // P2's Lipschitz partials, by pass parity: one per body tile, then one
                         // per heavy body (in the heavy list's order)
   double* p1sum[2];     // single slot: P1's slots totalled by P2's last workgroup
@@ -1043,10 +1043,12 @@ __global__ __launch_bounds__(kTC / R, (kP1MinWaves<TM, R>)) void kP1(const Args<
   constexpr bool kMixed = sizeof(TM) < sizeof(double);
   constexpr int NT = kTC / R;
   constexpr int NF = kMixed ? 6 : 3;  // stored frame components
-  // the contacts' body ids, which the rows of N gamma_k gather with: a single
-  // slot loads them before it takes the previous decision, so that the
-  // gathers of a fresh iterate's rows go out with the iterate's loads
-  auto loadIds = [&](const Tile tile, const Args<TM>& a, int (&ia)[R], int (&ib)[R]) {
+  // the contacts' body ids and both bodies' tmp (M^-1 D^T of the iterate the
+  // previous pass formed), from which the rows of N gamma_k are formed when
+  // the previous decision accepted that iterate: a single slot loads them
+  // before it takes the decision, so that they arrive while it decides
+  // (after a backtrack the gathered tmp is not used)
+  auto loadIds = [&](const Tile tile, const Args<TM>& a, int (&ia)[R], int (&ib)[R], V4<TM> (&tA)[R], V4<TM> (&tB)[R]) {
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       const long q = (long)tile.start + threadIdx.x + u * NT;
@@ -1054,8 +1056,17 @@ __global__ __launch_bounds__(kTC / R, (kP1MinWaves<TM, R>)) void kP1(const Args<
       ia[u] = gl(a.cA)[qf];
       ib[u] = gl(a.cB)[qf];
     }
+    // (rows that are not a contact gather body 0: the id arrays hold at
+    // least one entry, but maybe no contact)
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const bool own = (long)tile.start + threadIdx.x + u * NT < a.dims.nc;
+      tA[u] = ld4(a.tmp, own ? ia[u] : 0);
+      tB[u] = ld4(a.tmp, own ? ib[u] : 0);
+    }
   };
-  auto tileBody = [&](const Tile tile, const PassView& S, const Args<TM>& a, const int (&ia)[R], const int (&ib)[R]) {
+  auto tileBody = [&](const Tile tile, const PassView& S, const Args<TM>& a, const int (&ia)[R], const int (&ib)[R],
+                      const V4<TM> (&tA)[R], const V4<TM> (&tB)[R]) {
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
     const DV_GLOBAL double* Gk = gl(a.ring[0] + S.cur * a.ringStride);
@@ -1095,16 +1106,6 @@ __global__ __launch_bounds__(kTC / R, (kP1MinWaves<TM, R>)) void kP1(const Args<
       pb[u] = gl(a.posB)[qf];
     }
     if (fresh) {
-      // rows that are not a contact gather body 0 (the id arrays hold at
-      // least one entry, but maybe no contact)
-      V4<TM> tA[R], tB[R];
-#pragma unroll
-      for (int u = 0; u < R; ++u) {
-        const long q = (long)tile.start + threadIdx.x + u * NT;
-        const bool own = q < D.nc;
-        tA[u] = ld4(a.tmp, own ? ia[u] : 0);
-        tB[u] = ld4(a.tmp, own ? ib[u] : 0);
-      }
       DV_GLOBAL TM* Nw = gl(a.rn[0] + S.cur * a.ringStride);
 #pragma unroll
       for (int u = 0; u < R; ++u) {
@@ -1207,17 +1208,22 @@ __global__ __launch_bounds__(kTC / R, (kP1MinWaves<TM, R>)) void kP1(const Args<
   };
   if (T.single) {
     // one slot: the grid has a workgroup for every tile the slot's capacity
-    // can hold, so each takes one tile and nothing stays live across a tile
-    // loop (which held P1 above 80 VGPRs).  Tiles past the row count store
-    // only their partial sums, which no decision reads, so no branch waits
-    // on the tile count; workgroup 0 stores the decided state.
+    // can hold (the captured graph outlives changes of the row count), so
+    // each takes one tile and nothing stays live across a tile loop (which
+    // held P1 above 80 VGPRs).  A workgroup whose tile lies past the row
+    // count leaves once the count has arrived, before the decision's
+    // reductions (its partial sums are never read); workgroup 0 always
+    // stays, to store the decided state.  The count comes with the table
+    // entry, which the id loads wait for anyway.
     const Args<TM>& a = A[0];
     const Tile tile{0, (int)blockIdx.x * kTC};
     int ia[R], ib[R];
-    loadIds(tile, a, ia, ib);
+    V4<TM> tA[R], tB[R];
+    loadIds(tile, a, ia, ib, tA, tB);
+    if (blockIdx.x > 0 && (long)tile.start >= (long)a.dims.nc + a.dims.nbil) return;
     const PassView S = kFold ? decidePrevious(a, T, par) : loadView(T, 0, par ^ 1);
     if (viewDone(S)) return;
-    tileBody(tile, S, a, ia, ib);
+    tileBody(tile, S, a, ia, ib, tA, tB);
     return;
   }
   TILE_LOOP(T) {
@@ -1225,8 +1231,9 @@ __global__ __launch_bounds__(kTC / R, (kP1MinWaves<TM, R>)) void kP1(const Args<
     const PassView S = loadView(T, tile.scene, par ^ 1);
     if (viewDone(S)) continue;
     int ia[R], ib[R];
-    loadIds(tile, A[tile.scene], ia, ib);
-    tileBody(tile, S, A[tile.scene], ia, ib);
+    V4<TM> tA[R], tB[R];
+    loadIds(tile, A[tile.scene], ia, ib, tA, tB);
+    tileBody(tile, S, A[tile.scene], ia, ib, tA, tB);
   }
 }
 
