@@ -684,6 +684,7 @@ def main():
             "residual_max": float(np.max(r["residual"])),
             "steps_converged": int(np.sum(r["residual"] < TOL)),
             "us_per_iteration": us_per_iteration(r),
+            "schedule": r["schedule"],
         }
     # ratios over the same steps from the same state: the plain path runs the
     # first steps of the window only
