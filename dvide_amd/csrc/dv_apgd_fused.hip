@@ -275,7 +275,11 @@ __device__ __forceinline__ double waveMax(double v) {
 // combined in row order, so the result is that of one row per thread on
 // R * NT threads, bit for bit: the rows-per-thread schedule changes the
 // work per lane, not the arithmetic.
-template <int NT, int R, int NS>
+//
+// Slots outside kUsed are left out and given as 0: a caller whose build
+// never adds to a slot saves its wavefront reduction (the mixed contact
+// pass fills 3 of its 8 slots).
+template <int NT, int R, int NS, unsigned kUsed = 0xFFFFFFFFu>
 __device__ void blockReduceR(double (&acc)[R][NS], double* out) {
   constexpr int W = NT / 64;
   __shared__ double sh[R * W][NS];
@@ -284,6 +288,7 @@ __device__ void blockReduceR(double (&acc)[R][NS], double* out) {
   for (int u = 0; u < R; ++u) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
+      if (!((kUsed >> s) & 1u)) continue;
       double v = (s % kSlots == kSlots - 1) ? waveMax(acc[u][s]) : waveSum(acc[u][s]);
       if (lane == 0) sh[u * W + wave][s] = v;
     }
@@ -291,8 +296,11 @@ __device__ void blockReduceR(double (&acc)[R][NS], double* out) {
   __syncthreads();
   if (threadIdx.x < NS) {
     const int s = threadIdx.x;
-    double v = sh[0][s];
-    for (int w = 1; w < R * W; ++w) v = (s % kSlots == kSlots - 1) ? fmax(v, sh[w][s]) : v + sh[w][s];
+    double v = 0.0;
+    if ((kUsed >> s) & 1u) {
+      v = sh[0][s];
+      for (int w = 1; w < R * W; ++w) v = (s % kSlots == kSlots - 1) ? fmax(v, sh[w][s]) : v + sh[w][s];
+    }
     out[s] = v;
   }
   __syncthreads();
@@ -303,10 +311,10 @@ __device__ void blockReduce(double (&acc)[NS], double* out) {
   blockReduceR<NT, 1, NS>(reinterpret_cast<double(&)[1][NS]>(acc), out);
 }
 
-template <int NT, int R = 1>
+template <int NT, int R = 1, unsigned kUsed = 0xFFFFFFFFu>
 __device__ void writePartialsR(double (&acc)[R][kSlots], double* part) {
   __shared__ double tot[kSlots];
-  blockReduceR<NT, R, kSlots>(acc, tot);
+  blockReduceR<NT, R, kSlots, kUsed>(acc, tot);
   if (threadIdx.x < kSlots) gl(part)[threadIdx.x] = tot[threadIdx.x];
 }
 
@@ -360,7 +368,8 @@ __device__ void reduceAll(const double* part1, const double* part3, int ng, doub
 // and P2's Lipschitz partials this way, wherever it takes them from.  In a wider
 // workgroup the threads past kTP add nothing, so the totals are the same.
 constexpr int kTP = 256;
-template <int NS, int kStride = kSlots, int K = 8>
+// Slots outside kUsed are not read and are given as 0.
+template <int NS, int kStride = kSlots, int K = 8, unsigned kUsed = 0xFFFFFFFFu>
 __device__ void reduceSlots(const double* part, int s0, int ng, double (&out)[NS]) {
   constexpr int W = kTP / 64;
   __shared__ double sh[W][NS];
@@ -376,10 +385,12 @@ __device__ void reduceSlots(const double* part, int s0, int ng, double (&out)[NS
       const int w = base + (int)threadIdx.x + kTP * k;
       const bool on = mine && w < ng;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) x[k][s] = on ? gl(part)[(long)w * kStride + s0 + s] : 0.0;
+      for (int s = 0; s < NS; ++s)
+        x[k][s] = on && ((kUsed >> s) & 1u) ? gl(part)[(long)w * kStride + s0 + s] : 0.0;
     }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
+      if (!((kUsed >> s) & 1u)) continue;
       const bool isMax = s0 + s == kSlots - 1;
 #pragma unroll
       for (int k = 0; k < K; ++k) v[s] = isMax ? fmax(v[s], x[k][s]) : v[s] + x[k][s];
@@ -387,30 +398,39 @@ __device__ void reduceSlots(const double* part, int s0, int ng, double (&out)[NS
   }
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
+    if (!((kUsed >> s) & 1u)) continue;
     const double t = s0 + s == kSlots - 1 ? waveMax(v[s]) : waveSum(v[s]);
     if (lane == 0 && wave < W) sh[wave][s] = t;
   }
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    double t = sh[0][s];
-    for (int w = 1; w < W; ++w) t = s0 + s == kSlots - 1 ? fmax(t, sh[w][s]) : t + sh[w][s];
+    double t = 0.0;
+    if ((kUsed >> s) & 1u) {
+      t = sh[0][s];
+      for (int w = 1; w < W; ++w) t = s0 + s == kSlots - 1 ? fmax(t, sh[w][s]) : t + sh[w][s];
+    }
     out[s] = t;
   }
   __syncthreads();
 }
 
-// The totals the pass decision needs: P1's slots 0..5 and 7 (slot 6 is
-// unused) and, in slot kSlots + 6, P2's Lipschitz term (its partials per
-// body tile and per heavy body, packed in lq, nl of them); the other slots
-// are 0.  reducePass reduces P1's partials; passTotals takes them already
+// The slots of P1's partial sums each build adds to (the others stay zero).
+template <typename TM>
+constexpr unsigned kP1Slots = sizeof(TM) < sizeof(double) ? 0xB0u : 0xBFu;
+
+// The totals the pass decision needs: P1's slots (kP1Slots: 4, 5 and 7 in
+// the mixed build, 0..5 and 7 in the FP64 build) and, in slot kSlots + 6,
+// P2's Lipschitz term (its partials per body tile and per heavy body, packed
+// in lq, nl of them); the other slots are 0.  reducePass reduces P1's partials; passTotals takes them already
 // totalled (a single slot's P2 totals them), which keeps the registers of
 // the eight-slot reduction out of the kernels that decide a single slot.
+template <typename TM>
 __device__ void reducePass(const double* part1, int ng, const double* lq, int nl, double* out) {
 #pragma unroll
   for (int s = 0; s < 2 * kSlots; ++s) out[s] = 0.0;
   double t1[kSlots], t3[1];
-  reduceSlots<kSlots>(part1, 0, ng, t1);
+  reduceSlots<kSlots, kSlots, 8, kP1Slots<TM>>(part1, 0, ng, t1);
   reduceSlots<1, 1>(lq, 0, nl, t3);
 #pragma unroll
   for (int s = 0; s < kSlots; ++s) out[s] = t1[s];
@@ -663,7 +683,7 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
     const Args<TM>& a = A[0];
     const int par = sIdx ^ 1;
     double t1[kSlots];
-    reduceSlots<kSlots, kSlots, 2>(a.part1[par], 0, a.nTiles, t1);  // (two tiles' loads at a time: the same order)
+    reduceSlots<kSlots, kSlots, 2, kP1Slots<TM>>(a.part1[par], 0, a.nTiles, t1);  // (two tiles' loads at a time: the same order)
     if (threadIdx.x < kSlots) gl(a.p1sum[par])[threadIdx.x] = t1[threadIdx.x];
     return;
   }
@@ -794,7 +814,7 @@ __global__ __launch_bounds__(kTB) void kBody(const Args<TM>* __restrict__ A, Tab
       }
     }
     __shared__ double tot[kSlots];
-    blockReduce<kTB, kSlots>(acc, tot);
+    blockReduceR<kTB, 1, kSlots, 0x07u>(reinterpret_cast<double(&)[1][kSlots]>(acc), tot);
     if (threadIdx.x == 0) {
       V4<TM> out;
       out.x = mi * (TM)tot[0];
@@ -1204,7 +1224,9 @@ __global__ __launch_bounds__(kTC / R, (kP1MinWaves<TM, R>)) void kP1(const Args<
         storeJ(a, (int)q, pb[u], F, (TM)gn[0], (TM)gn[1], (TM)gn[2]);
       }
     }
-    writePartialsR<NT, R>(acc, TILE_PART(a.part1[par], tile));
+    // the slots this build adds to (the others are stored as zeros): the
+    // mixed solver's Lipschitz test takes only |Delta|^2 from P1
+    writePartialsR<NT, R, kP1Slots<TM>>(acc, TILE_PART(a.part1[par], tile));
   };
   if (T.single) {
     // one slot: the grid has a workgroup for every tile the slot's capacity
@@ -1288,7 +1310,7 @@ __global__ __launch_bounds__(kTP) void kDecide(const Args<TM>* __restrict__ A, T
   ApgdState S = STATE(T, blockIdx.x, par ^ 1);
   if (!S.done) {
     double tot[2 * kSlots];
-    reducePass(a.part1[par], a.nTiles, a.lq[par], a.nBodyTiles + gl(a.counters)[2], tot);
+    reducePass<TM>(a.part1[par], a.nTiles, a.lq[par], a.nBodyTiles + gl(a.counters)[2], tot);
     decideStep<kMixed>(S, tot, a.dims);
   }
   if (threadIdx.x == 0) {
@@ -2140,6 +2162,9 @@ void solveAll(Engine& E, const std::vector<Scene*>& scenes, hipStream_t st, int 
   // a measured chunk counts when no slot finished during it: then every
   // active slot ran all of its passes
   int doneBefore = 0;
+  // slots already retired while others iterate (several slots only)
+  std::vector<char> retired(n, 0);
+  int nRetired = 0;
   double activeRows = (double)rows;
   launch(cur, false);
   for (;;) {
@@ -2173,6 +2198,23 @@ void solveAll(Engine& E, const std::vector<Scene*>& scenes, hipStream_t st, int 
     doneBefore = doneNow;
     activeRows = rowsLeft;
     if (doneNow == n) break;
+    // Once a quarter of the slots have finished since the last retirement,
+    // retire them while the others iterate: the pass kernels then loop over
+    // the active slots' tiles only, instead of reading the view of every
+    // finished slot's tiles on every pass.  A finished slot's state is
+    // frozen, so nothing changes in any result.  The retirement waits for
+    // the chunk queued ahead (it runs on the same stream) and rebuilds the
+    // tile tables in place, so the captured chunk stays valid.
+    if (n > 1 && doneNow - nRetired >= std::max(4, n / 4)) {
+      std::vector<int> fin;
+      for (int i = 0; i < n; ++i)
+        if (last[i].done && !retired[i]) {
+          fin.push_back(i);
+          retired[i] = 1;
+        }
+      nRetired += (int)fin.size();
+      retireSlots<TM>(E, fin, st);
+    }
     if (E.launched > passCap) throw DvError{DV_ERR_DIVERGED, "APGD exceeded its pass budget"};
     cur ^= 1;
     curTimed = false;
@@ -2183,8 +2225,9 @@ void solveAll(Engine& E, const std::vector<Scene*>& scenes, hipStream_t st, int 
       curTimed = timed;
     }
   }
-  std::vector<int> slots(n);
-  for (int i = 0; i < n; ++i) slots[i] = i;
+  std::vector<int> slots;
+  for (int i = 0; i < n; ++i)
+    if (!retired[i]) slots.push_back(i);
   retireSlots<TM>(E, slots, st);
 }
 
