@@ -75,7 +75,6 @@ namespace dv {
 
 struct ApgdState {
   double L, t, theta, beta, best, objective;
-  double sum[2 * kSlots];
   int it, bt, backtracks, restart;
   int cur, prev, nxt;
   int hatSrc, done, converged, passes;
@@ -902,7 +901,6 @@ __global__ __launch_bounds__(kTD) void kSetupDecide(const Args<TM>* __restrict__
     S.beta = 0.0;
     S.best = 10e30;
     S.objective = 0;
-    for (int k = 0; k < 2 * kSlots; ++k) S.sum[k] = 0;
     S.it = 0; S.bt = 0; S.backtracks = 0; S.restart = 1;
     S.cur = 0; S.prev = 0; S.nxt = 1; S.hatSrc = -1;  // -1: gHat (ones)
     S.pendingRes = 0;  // gamma_0 is not an accepted iterate (APGD.cu:228-230)
@@ -924,7 +922,6 @@ __global__ __launch_bounds__(kTD) void kSetupDecide(const Args<TM>* __restrict__
 
 template <bool kMixed>
 __device__ void decideStep(ApgdState& S, const double* tot, const SolveDims& D) {
-  for (int k = 0; k < 2 * kSlots; ++k) S.sum[k] = tot[k];
   const double* s1 = tot;           // P1 slots
   const double* s3 = tot + kSlots;  // slot 6: P2's Lipschitz term
   S.passes++;
