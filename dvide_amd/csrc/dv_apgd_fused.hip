@@ -1027,8 +1027,8 @@ __device__ __forceinline__ void frameFrom(const float (&c)[6], int, float (&F)[9
 // given a third tile then run it beside the other two instead of after one
 // of them (P1 11% faster at 263k contacts, 2% slower at 256k, where no CU
 // takes a third tile).  The FP64 instance, which holds twice the registers
-// per value and, with the fresh iterate's N formed here, needs 112 VGPRs, is
-// held to four wavefronts so that it does not spill.
+// per value, needs 94 VGPRs (five wavefronts); it is held to no fewer than
+// four, so that it does not spill.
 template <typename TM, int R>
 constexpr int kP1MinWaves = R == 1 ? (sizeof(TM) < sizeof(double) ? 6 : 4) : 1;
 
@@ -1065,6 +1065,31 @@ __global__ __launch_bounds__(kTC / R, (kP1MinWaves<TM, R>)) void kP1(const Args<
   // the previous decision accepted that iterate: a single slot loads them
   // before it takes the decision, so that they arrive while it decides
   // (after a backtrack the gathered tmp is not used)
+  // the loads that do not depend on the pass view (r, the frame, the B-side
+  // slot), issued before the view (and a single slot's decision) arrives
+  struct Pre {
+    TM rv[R][3], fc[R][6];
+    int pb[R];
+  };
+  auto loadPre = [&](const Tile tile, const Args<TM>& a, Pre& pr) {
+    const SolveDims& D = a.dims;
+    const long items = (long)D.nc + D.nbil;
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const long q = (long)tile.start + threadIdx.x + u * NT;
+      const bool valid = q < items, bil = q >= D.nc;
+      const long qf = (valid && !bil) ? q : 0;
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr) {
+        const bool on = valid && (rr == 0 || !bil);
+        const long f = on ? (bil ? BROW(a, q - D.nc) : ROW(a, rr, q)) : 0;
+        pr.rv[u][rr] = gl(a.r)[f];
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) pr.fc[u][k] = k < NF ? gl(a.frame)[k * a.cap + qf] : (TM)0;
+      pr.pb[u] = gl(a.posB)[qf];
+    }
+  };
   auto loadIds = [&](const Tile tile, const Args<TM>& a, int (&ia)[R], int (&ib)[R], V4<TM> (&tA)[R], V4<TM> (&tB)[R]) {
 #pragma unroll
     for (int u = 0; u < R; ++u) {
@@ -1083,7 +1108,7 @@ __global__ __launch_bounds__(kTC / R, (kP1MinWaves<TM, R>)) void kP1(const Args<
     }
   };
   auto tileBody = [&](const Tile tile, const PassView& S, const Args<TM>& a, const int (&ia)[R], const int (&ib)[R],
-                      const V4<TM> (&tA)[R], const V4<TM> (&tB)[R]) {
+                      const V4<TM> (&tA)[R], const V4<TM> (&tB)[R], const Pre& pr) {
     const SolveDims& D = a.dims;
     const long items = (long)D.nc + D.nbil;
     const DV_GLOBAL double* Gk = gl(a.ring[0] + S.cur * a.ringStride);
@@ -1101,26 +1126,23 @@ __global__ __launch_bounds__(kTC / R, (kP1MinWaves<TM, R>)) void kP1(const Args<
     // after; read back otherwise (the flag is the same in every workgroup)
     const bool fresh = (S.flags & kViewFresh) != 0;
     double gk[R][3], gm[R][3];
-    TM rv[R][3], nk[R][3], nm[R][3], fc[R][6];
-    int pb[R];  // the contacts' B-side slots, loaded with the rest
+    TM nk[R][3], nm[R][3];
+    const auto& rv = pr.rv;
+    const auto& fc = pr.fc;
+    const auto& pb = pr.pb;
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       const long q = (long)tile.start + threadIdx.x + u * NT;
       const bool valid = q < items, bil = q >= D.nc;
-      const long qf = (valid && !bil) ? q : 0;
 #pragma unroll
       for (int rr = 0; rr < 3; ++rr) {
         const bool on = valid && (rr == 0 || !bil);
         const long f = on ? (bil ? BROW(a, q - D.nc) : ROW(a, rr, q)) : 0;
         gk[u][rr] = Gk[f];
-        rv[u][rr] = gl(a.r)[f];
         gm[u][rr] = Gkm1[f];  // read even after a restart (unused then)
         nm[u][rr] = Nkm1[f];
         nk[u][rr] = (TM)0;
       }
-#pragma unroll
-      for (int k = 0; k < 6; ++k) fc[u][k] = k < NF ? gl(a.frame)[k * a.cap + qf] : (TM)0;
-      pb[u] = gl(a.posB)[qf];
     }
     if (fresh) {
       DV_GLOBAL TM* Nw = gl(a.rn[0] + S.cur * a.ringStride);
@@ -1240,9 +1262,11 @@ __global__ __launch_bounds__(kTC / R, (kP1MinWaves<TM, R>)) void kP1(const Args<
     V4<TM> tA[R], tB[R];
     loadIds(tile, a, ia, ib, tA, tB);
     if (blockIdx.x > 0 && (long)tile.start >= (long)a.dims.nc + a.dims.nbil) return;
+    Pre pr;
+    loadPre(tile, a, pr);
     const PassView S = kFold ? decidePrevious(a, T, par) : loadView(T, 0, par ^ 1);
     if (viewDone(S)) return;
-    tileBody(tile, S, a, ia, ib, tA, tB);
+    tileBody(tile, S, a, ia, ib, tA, tB, pr);
     return;
   }
   TILE_LOOP(T) {
@@ -1252,7 +1276,9 @@ __global__ __launch_bounds__(kTC / R, (kP1MinWaves<TM, R>)) void kP1(const Args<
     int ia[R], ib[R];
     V4<TM> tA[R], tB[R];
     loadIds(tile, A[tile.scene], ia, ib, tA, tB);
-    tileBody(tile, S, A[tile.scene], ia, ib, tA, tB);
+    Pre pr;
+    loadPre(tile, A[tile.scene], pr);
+    tileBody(tile, S, A[tile.scene], ia, ib, tA, tB, pr);
   }
 }
 
