@@ -1656,6 +1656,7 @@ TuneClass* chooseSchedule(Engine& E, long rows, bool allowTuning) {
     E.sched.rpt = E.forcedRpt;
     return nullptr;
   }
+  if (!tuningEnabled()) return nullptr;  // the first candidate, whatever the schedule file holds
   Tuner& T = tuner();
   std::lock_guard<std::mutex> lock(T.m);
   int bucket = 0;
@@ -1667,7 +1668,7 @@ TuneClass* chooseSchedule(Engine& E, long rows, bool allowTuning) {
     E.sched = c.sched;
     return nullptr;
   }
-  if (!allowTuning || !tuningEnabled()) return nullptr;
+  if (!allowTuning) return nullptr;
   if ((int)c.score.size() != n) c.score.assign(n, -1.0);
   E.tuneKey = key;
   return &c;
