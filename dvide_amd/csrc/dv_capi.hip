@@ -12,8 +12,10 @@
 #include <cstring>
 #include <deque>
 #include <exception>
+#include <fstream>
 #include <mutex>
 #include <new>
+#include <sstream>
 #include <thread>
 
 struct dv_scene {
@@ -95,6 +97,8 @@ void initialize(Scene& s) {
   for (auto& e : s.ev)
     if (!e) DV_HIP(hipEventCreate(&e));
   if (!s.pinned) DV_HIP(hipHostMalloc(&s.pinned, 16 * sizeof(int), hipHostMallocDefault));
+  s.nUser = (int)s.hMass.size();
+  if (s.nNodes) shellAppendPoints(s);  // the shells' contact points follow the caller's bodies
   const int nb = (int)s.hMass.size();
   s.nb = nb;
   // System::initializeSystem (System.cu:344-424): f = m g (0 if fixed), M^-1 = 1/m (0 if fixed)
@@ -114,12 +118,16 @@ void initialize(Scene& s) {
   s.fContact.reserve(3 * (size_t)std::max(nb, 1));
   s.tmp.reserve(3 * (size_t)std::max(nb, 1));
   fillDevice(s, s.fContact.p, 3L * nb, 0.0);
+  if (s.nNodes) shellPlacePoints(s);
   // bilateral DOF constraints (System::processConstraints, System.cu:303-340)
   s.nbil = (int)s.hBilA.size();
   s.hBilP0.assign(s.nbil, 0.0);
   for (int i = 0; i < s.nbil; ++i) {
-    if (s.hBilA[i] < 0 || s.hBilA[i] >= 3 * nb || s.hBilB[i] >= 3 * nb)
+    const int ndof = 3 * s.nUser + 9 * s.nNodes;
+    if (s.hBilA[i] < 0 || s.hBilA[i] >= ndof || s.hBilB[i] >= ndof)
       throw DvError{DV_ERR_ARG, "bilateral constraint DOF out of range"};
+    if (s.hBilA[i] >= 3 * s.nUser || s.hBilB[i] >= 3 * s.nUser)
+      throw DvError{DV_ERR_UNSUPPORTED, "bilateral constraints on shell DOFs (the shells are fixed)"};
     if (s.hBilB[i] < 0) s.hBilP0[i] = s.hPos[s.hBilA[i]];
   }
   upload(s.bilA, s.hBilA, st);
@@ -140,10 +148,11 @@ void initialize(Scene& s) {
   }
   upload(s.bilIncStart, bs, st);
   upload(s.bilInc, bl, st);
-  // gamma_d starts as zeros, one per DOF (System.cu:185)
-  s.gamma.reserve(3 * (size_t)std::max(nb, 1));
-  fillDevice(s, s.gamma.p, 3L * nb, 0.0);
-  s.gammaLen = 3 * nb;
+  // gamma_d starts as zeros, one per DOF (System.cu:185), the shells' included
+  const long ndof = 3L * s.nUser + 9L * s.nNodes;
+  s.gamma.reserve((size_t)std::max(ndof, 1L));
+  fillDevice(s, s.gamma.p, ndof, 0.0);
+  s.gammaLen = (int)ndof;
   collideInit(s);
   DV_HIP(hipStreamSynchronize(st));
   s.initialized = true;
@@ -286,12 +295,13 @@ int dv_scene_destroy(dv_scene* h) {
   if (s.stream) (void)hipStreamSynchronize(s.stream);
   dv::DevBuf<double>* dbufs[] = {&s.p, &s.v, &s.f, &s.k, &s.fContact, &s.tmp, &s.minv, &s.geom, &s.bilVel,
                                  &s.bilT0, &s.bilP0, &s.cellPosR, &s.frame, &s.gap, &s.loc, &s.r,
-                                 &s.b, &s.gamma, &s.work, &s.partials};
+                                 &s.b, &s.gamma, &s.work, &s.partials, &s.shellNodes, &s.ptS, &s.shellF};
   for (auto* b : dbufs) b->release();
   dv::DevBuf<int>* ibufs[] = {&s.family, &s.smallIds, &s.bigIds, &s.bilA, &s.bilB, &s.bilIncStart,
                               &s.bilInc, &s.cellItems, &s.cntUp, &s.cntDown, &s.cStart, &s.iStart,
                               &s.scanTmp, &s.counters, &s.hostTotals, &s.cA, &s.cB, &s.inc, &s.posB,
-                              &s.heavy, &s.heavyPos, &s.bigChunks};
+                              &s.heavy, &s.heavyPos, &s.bigChunks, &s.shellConn, &s.ptEl, &s.nodeIncStart,
+                              &s.nodeInc};
   for (auto* b : ibufs) b->release();
   s.slotOf.release(); s.slotCount.release(); s.slotStart.release(); s.slotFill.release();
   s.frame32.release(); s.work32.release();
@@ -342,6 +352,97 @@ int dv_add_bilateral_dof(dv_scene* h, int dofA, int dofB, double velocity, doubl
     s.hBilT0.push_back(start_time);
     return DV_OK;
   });
+}
+
+int dv_add_shell_mesh(dv_scene* h, int n_nodes, const double* nodes, int n_elements, const int* connectivity,
+                      const double* geometry, int points_per_side, const double* fext) {
+  return dv::guarded(h, [&](Scene& s) {
+    if (s.initialized) return dv::fail(s, DV_ERR_STATE, "shell meshes must be added before initializeSystem");
+    if (s.nNodes) return dv::fail(s, DV_ERR_UNSUPPORTED, "one shell mesh per scene");
+    if (n_nodes <= 0 || n_elements <= 0 || !nodes || !connectivity || !geometry)
+      return dv::fail(s, DV_ERR_ARG, "bad shell mesh arrays");
+    if (points_per_side < 2) return dv::fail(s, DV_ERR_ARG, "points per element side must be at least 2");
+    for (int e = 0; e < n_elements; ++e) {
+      for (int q = 0; q < 4; ++q)
+        if (connectivity[4 * e + q] < 0 || connectivity[4 * e + q] >= n_nodes)
+          return dv::fail(s, DV_ERR_ARG, "shell connectivity names a node out of range");
+      for (int q = 0; q < 3; ++q)
+        if (!(geometry[4 * e + q] > 0)) return dv::fail(s, DV_ERR_ARG, "shell sides and thickness must be positive");
+    }
+    for (long i = 0; i < 9L * n_nodes; ++i)
+      if (!std::isfinite(nodes[i]) || (fext && !std::isfinite(fext[i])))
+        return dv::fail(s, DV_ERR_ARG, "non-finite shell node data");
+    s.nNodes = n_nodes;
+    s.hShellNodes.assign(nodes, nodes + 9L * n_nodes);
+    if (fext) s.hShellFext.assign(fext, fext + 9L * n_nodes);
+    else s.hShellFext.assign(9L * n_nodes, 0.0);
+    s.hShellConn.assign(connectivity, connectivity + 4L * n_elements);
+    s.hShellGeom.clear();
+    for (int e = 0; e < n_elements; ++e)
+      for (int q = 0; q < 3; ++q) s.hShellGeom.push_back(geometry[4 * e + q]);
+    s.shellPoints = points_per_side;
+    return DV_OK;
+  });
+}
+
+int dv_import_mesh(dv_scene* h, const char* path, double stiffness, int points_per_side, double pressure) {
+  (void)stiffness;  // the elastic and pressure forces act on free shell DOFs only
+  (void)pressure;
+  if (!h) return DV_ERR_ARG;
+  Scene& s = h->s;
+  if (!path) return dv::fail(s, DV_ERR_ARG, "null path");
+  std::ifstream in(path);
+  if (!in) return dv::fail(s, DV_ERR_ARG, (std::string("cannot open ") + path).c_str());
+  // one record per line, fields separated by commas or blanks
+  std::string line;
+  auto next = [&](std::istringstream& ss) {
+    if (!std::getline(in, line)) return false;
+    std::replace(line.begin(), line.end(), ',', ' ');
+    ss.clear();
+    ss.str(line);
+    return true;
+  };
+  std::istringstream ss;
+  long nn = 0, ne = 0, nM = 0, nInv = 0;
+  if (!next(ss) || !(ss >> nn >> ne >> nM >> nInv) || nn <= 0 || ne <= 0 || nM < 0 || nInv < 0)
+    return dv::fail(s, DV_ERR_ARG, "bad mesh header (nodes, shells, M entries, M^-1 entries)");
+  std::vector<double> nodes(9 * nn), fext(9 * nn), geom(4 * ne);
+  std::vector<int> conn(4 * ne);
+  for (long i = 0; i < 3 * nn; ++i)
+    if (!next(ss) || !(ss >> nodes[3 * i] >> nodes[3 * i + 1] >> nodes[3 * i + 2]))
+      return dv::fail(s, DV_ERR_ARG, "bad mesh node line");
+  for (long e = 0; e < ne; ++e)
+    if (!next(ss) || !(ss >> conn[4 * e] >> conn[4 * e + 1] >> conn[4 * e + 2] >> conn[4 * e + 3]))
+      return dv::fail(s, DV_ERR_ARG, "bad mesh connectivity line");
+  for (long e = 0; e < ne; ++e) {  // materials (used by the elastic forces only)
+    double m[4];
+    if (!next(ss) || !(ss >> m[0] >> m[1] >> m[2] >> m[3])) return dv::fail(s, DV_ERR_ARG, "bad mesh material line");
+  }
+  for (long e = 0; e < ne; ++e)
+    if (!next(ss) || !(ss >> geom[4 * e] >> geom[4 * e + 1] >> geom[4 * e + 2] >> geom[4 * e + 3]))
+      return dv::fail(s, DV_ERR_ARG, "bad mesh geometry line");
+  for (long i = 0; i < 36 * ne; ++i) {  // the element-to-DOF map (used by the elastic forces only)
+    long m;
+    if (!next(ss) || !(ss >> m)) return dv::fail(s, DV_ERR_ARG, "bad mesh map line");
+  }
+  for (long i = 0; i < 9 * nn; ++i)
+    if (!next(ss) || !(ss >> fext[i])) return dv::fail(s, DV_ERR_ARG, "bad mesh external force line");
+  for (long i = 0; i < nM; ++i) {  // M: the shell's momentum, which a fixed shell does not have
+    long r, c;
+    double v;
+    if (!next(ss) || !(ss >> r >> c >> v)) return dv::fail(s, DV_ERR_ARG, "bad mesh mass line");
+  }
+  for (long i = 0; i < nInv; ++i) {
+    long r, c;
+    double v;
+    if (!next(ss) || !(ss >> r >> c >> v)) return dv::fail(s, DV_ERR_ARG, "bad mesh inverse-mass line");
+    if (v != 0)
+      return dv::fail(s, DV_ERR_UNSUPPORTED,
+                      "the mesh's inverse mass is not zero: free shells need their elastic forces, which this "
+                      "library does not compute (only fixed shells are supported)");
+  }
+  return dv_add_shell_mesh(h, (int)nn, nodes.data(), (int)ne, conn.data(), geom.data(), points_per_side,
+                           fext.data());
 }
 
 int dv_set_bins_per_axis(dv_scene* h, int bx, int by, int bz) {
@@ -446,16 +547,26 @@ int dv_step(dv_scene* h) {
   });
 }
 
+// the caller's bodies (after initialisation the host arrays also hold the
+// shells' contact points)
+static int userBodies(const Scene& s) { return s.initialized ? s.nUser : (int)s.hMass.size(); }
+
 int dv_num_bodies(const dv_scene* h, int* n) {
   if (!h || !n) return DV_ERR_ARG;
-  *n = (int)h->s.hMass.size();
+  *n = userBodies(h->s);
+  return DV_OK;
+}
+
+int dv_num_shell_nodes(const dv_scene* h, int* n) {
+  if (!h || !n) return DV_ERR_ARG;
+  *n = h->s.nNodes;
   return DV_OK;
 }
 
 int dv_get_bodies(const dv_scene* h, double* mass, double* geom, int* fixed, int* family, size_t n) {
   if (!h) return DV_ERR_ARG;
   const Scene& s = h->s;
-  const size_t m = std::min(n, s.hMass.size());
+  const size_t m = std::min(n, (size_t)userBodies(s));
   for (size_t i = 0; i < m; ++i) {
     if (mass) mass[i] = s.hMass[i];
     if (fixed) fixed[i] = s.hFixed[i];
@@ -483,22 +594,39 @@ int dv_get_vector(dv_scene* h, int which, double* out, size_t n, size_t* len) {
     if (!s.initialized) return dv::fail(s, DV_ERR_STATE, "not initialized");
     const double* src = nullptr;
     size_t L = 0;
+    // DOF vectors: the caller's bodies, then the shells' 9 DOFs per node
+    const size_t nBody = 3 * (size_t)s.nUser, nShell = 9 * (size_t)s.nNodes;
+    bool dof = true;
     switch (which) {
-      case DV_VEC_P: src = s.p.p; L = 3 * (size_t)s.nb; break;
-      case DV_VEC_V: src = s.v.p; L = 3 * (size_t)s.nb; break;
-      case DV_VEC_GAMMA: src = s.gamma.p; L = (size_t)s.gammaLen; break;
-      case DV_VEC_F_CONTACT: src = s.fContact.p; L = 3 * (size_t)s.nb; break;
-      case DV_VEC_R: src = s.r.p; L = (size_t)s.R; break;
-      case DV_VEC_K: src = s.k.p; L = 3 * (size_t)s.nb; break;
-      case DV_VEC_B: src = s.b.p; L = (size_t)s.R; break;
+      case DV_VEC_P: src = s.p.p; break;
+      case DV_VEC_V: src = s.v.p; break;
+      case DV_VEC_F_CONTACT: src = s.fContact.p; break;
+      case DV_VEC_K:
+        if (nShell) return dv::fail(s, DV_ERR_UNSUPPORTED, "k of a scene with shells (their elastic forces are not computed)");
+        src = s.k.p;
+        break;
+      case DV_VEC_GAMMA: src = s.gamma.p; L = (size_t)s.gammaLen; dof = false; break;
+      case DV_VEC_R: src = s.r.p; L = (size_t)s.R; dof = false; break;
+      case DV_VEC_B: src = s.b.p; L = (size_t)s.R; dof = false; break;
       default: return dv::fail(s, DV_ERR_ARG, "unknown vector");
     }
+    if (dof) L = nBody + nShell;
     if (len) *len = L;
-    size_t m = std::min(n, L);
-    if (out && m) {
-      DV_HIP(hipMemcpyAsync(out, src, sizeof(double) * m, hipMemcpyDeviceToHost, s.stream));
-      DV_HIP(hipStreamSynchronize(s.stream));
+    if (!out || !n) return DV_OK;
+    const size_t m = std::min(n, dof ? nBody : L);
+    if (m) DV_HIP(hipMemcpyAsync(out, src, sizeof(double) * m, hipMemcpyDeviceToHost, s.stream));
+    std::vector<double> sh;
+    if (dof && n > nBody && nShell) {  // the shell DOFs: fixed nodes, no velocity
+      if (which == DV_VEC_P) sh = s.hShellNodes;
+      else if (which == DV_VEC_V) sh.assign(nShell, 0.0);
+      else {
+        sh.resize(nShell);
+        dv::shellForces(s, s.shellF.p);
+        DV_HIP(hipMemcpyAsync(sh.data(), s.shellF.p, sizeof(double) * nShell, hipMemcpyDeviceToHost, s.stream));
+      }
     }
+    DV_HIP(hipStreamSynchronize(s.stream));
+    if (!sh.empty()) std::copy(sh.begin(), sh.begin() + std::min(nShell, n - nBody), out + nBody);
     return DV_OK;
   });
 }
@@ -507,7 +635,8 @@ int dv_set_vector(dv_scene* h, int which, const double* in, size_t n) {
   return dv::guarded(h, [&](Scene& s) {
     if (!s.initialized) return dv::fail(s, DV_ERR_STATE, "not initialized");
     double* dst = nullptr;
-    size_t L = 3 * (size_t)s.nb;
+    const size_t nBody = 3 * (size_t)s.nUser, nShell = 9 * (size_t)s.nNodes;
+    size_t L = nBody + nShell;
     switch (which) {
       case DV_VEC_P: dst = s.p.p; break;
       case DV_VEC_V: dst = s.v.p; break;
@@ -515,6 +644,12 @@ int dv_set_vector(dv_scene* h, int which, const double* in, size_t n) {
       default: return dv::fail(s, DV_ERR_ARG, "only p, v and gamma can be written");
     }
     if (n != L || !in) return dv::fail(s, DV_ERR_ARG, "vector length mismatch");
+    if (which != DV_VEC_GAMMA && nShell) {  // the shells are fixed: their DOFs must keep their values
+      for (size_t i = 0; i < nShell; ++i)
+        if (in[nBody + i] != (which == DV_VEC_P ? s.hShellNodes[i] : 0.0))
+          return dv::fail(s, DV_ERR_UNSUPPORTED, "the shell DOFs are fixed");
+      n = nBody;
+    }
     if (n) {
       DV_HIP(hipMemcpyAsync(dst, in, sizeof(double) * n, hipMemcpyHostToDevice, s.stream));
       DV_HIP(hipStreamSynchronize(s.stream));
@@ -556,12 +691,6 @@ int dv_get_jacobian(dv_scene* h, int* I, int* J, double* V, size_t cap, size_t* 
     const int nc = s.nc, nbil = (int)s.hBilA.size();
     size_t bilNnz = 0;
     for (int q = 0; q < nbil; ++q) bilNnz += s.hBilB[q] >= 0 ? 2 : 1;
-    const size_t total = bilNnz + 18 * (size_t)nc;
-    if (nnz) *nnz = total;
-    if (rows) *rows = 3 * nc + nbil;
-    if (cols) *cols = 3 * s.nb;
-    if (!I || !J || !V) return DV_OK;
-    if (cap < total) return dv::fail(s, DV_ERR_ARG, "buffers smaller than nnz");
     std::vector<double> fr(9 * (size_t)std::max(s.ncCap, 1));
     std::vector<int> a(std::max(nc, 1)), b(std::max(nc, 1));
     if (nc) {
@@ -570,20 +699,45 @@ int dv_get_jacobian(dv_scene* h, int* I, int* J, double* V, size_t cap, size_t* 
       DV_HIP(hipMemcpyAsync(b.data(), s.cB.p, sizeof(int) * nc, hipMemcpyDeviceToHost, s.stream));
       DV_HIP(hipStreamSynchronize(s.stream));
     }
+    // a side of a contact row: a body's 3 DOFs, or the 36 DOFs of a shell
+    // point's element, node by node (position, x-gradient, y-gradient, each
+    // x, y, z) weighted by the shape functions (System.cu:785-800, 904-961)
+    const int nU = s.nUser;
+    auto width = [&](int g) { return g < nU ? 3 : 36; };
+    size_t total = bilNnz;
+    for (int c = 0; c < nc; ++c) total += 3 * (size_t)(width(a[c]) + width(b[c]));
+    if (nnz) *nnz = total;
+    if (rows) *rows = 3 * nc + nbil;
+    if (cols) *cols = 3 * nU + 9 * s.nNodes;
+    if (!I || !J || !V) return DV_OK;
+    if (cap < total) return dv::fail(s, DV_ERR_ARG, "buffers smaller than nnz");
     size_t o = 0;
     for (int q = 0; q < nbil; ++q) {  // System.cu:622-638
       I[o] = q; J[o] = s.hBilA[q]; V[o] = 1.0; ++o;
       if (s.hBilB[q] >= 0) { I[o] = q; J[o] = s.hBilB[q]; V[o] = -1.0; ++o; }
     }
+    auto side = [&](int g, int row, const double* val, double sign) {
+      if (g < nU) {
+        for (int q = 0; q < 3; ++q) { I[o] = row; J[o] = 3 * g + q; V[o] = sign * val[q]; ++o; }
+        return;
+      }
+      const int j = g - nU, e = s.hPtEl[j];
+      const double* S = &s.hPtS[12 * (size_t)j];
+      for (int q = 0; q < 4; ++q)
+        for (int m = 0; m < 9; ++m) {
+          I[o] = row;
+          J[o] = 3 * nU + 9 * s.hShellConn[4 * e + q] + m;
+          V[o] = sign * (S[3 * q + m / 3] * val[m % 3]);
+          ++o;
+        }
+    };
     for (int c = 0; c < nc; ++c) {
       for (int rr = 0; rr < 3; ++rr) {
         const int row = nbil + 3 * c + rr;
-        for (int q = 0; q < 3; ++q) {
-          I[o] = row; J[o] = 3 * a[c] + q; V[o] = fr[(size_t)(3 * rr + q) * s.ncCap + c]; ++o;
-        }
-        for (int q = 0; q < 3; ++q) {
-          I[o] = row; J[o] = 3 * b[c] + q; V[o] = -fr[(size_t)(3 * rr + q) * s.ncCap + c]; ++o;
-        }
+        double val[3];
+        for (int q = 0; q < 3; ++q) val[q] = fr[(size_t)(3 * rr + q) * s.ncCap + c];
+        side(a[c], row, val, 1.0);
+        side(b[c], row, val, -1.0);
       }
     }
     return DV_OK;
@@ -627,11 +781,13 @@ int dv_energies(dv_scene* h, double out[3]) {
     DV_HIP(hipMemcpyAsync(mi.data(), s.minv.p, sizeof(double) * s.nb, hipMemcpyDeviceToHost, s.stream));
     DV_HIP(hipStreamSynchronize(s.stream));
     double pe = 0, ke = 0;
-    for (size_t i = 0; i < n3; ++i) {
+    for (size_t i = 0; i < 3 * (size_t)s.nUser; ++i) {
       pe += f[i] * p[i];
       const double m = mi[i / 3] != 0 ? 1.0 / mi[i / 3] : 0.0;  // multiplyByMass, System.cu:1411-1417
       ke += v[i] * (m * v[i]);
     }
+    // the fixed shells: f is their external force, v zero (no kinetic energy)
+    for (size_t i = 0; i < s.hShellFext.size(); ++i) pe += s.hShellFext[i] * s.hShellNodes[i];
     out[0] = -pe;
     out[1] = 0.5 * ke;
     out[2] = 0.0;

@@ -339,10 +339,12 @@ __global__ void zeroInt(int* x, int n) {
 }  // namespace
 
 void collideInit(Scene& s) {
-  // classify: spheres up to 4x the median sphere radius are hashed, the rest
-  // (boxes, large spheres) are tested against every geometry
+  // classify: spheres up to 4x the median radius of the caller's spheres are
+  // hashed, the rest (boxes, large spheres) are tested against every
+  // geometry; the shells' contact points (bodies from nUser on, spheres of
+  // half the shell's thickness) are always hashed
   std::vector<double> radii;
-  for (int i = 0; i < s.nb; ++i)
+  for (int i = 0; i < s.nUser; ++i)
     if (s.hGeom[3 * i + 1] == 0) radii.push_back(s.hGeom[3 * i]);
   double rmed = 0;
   if (!radii.empty()) {
@@ -354,7 +356,7 @@ void collideInit(Scene& s) {
   double rmax = 0;
   for (int i = 0; i < s.nb; ++i) {
     bool sphere = s.hGeom[3 * i + 1] == 0;
-    if (sphere && s.hGeom[3 * i] <= 4.0 * rmed) {
+    if (sphere && (i >= s.nUser || s.hGeom[3 * i] <= 4.0 * rmed)) {
       small.push_back(i);
       rmax = std::max(rmax, s.hGeom[3 * i]);
     } else {

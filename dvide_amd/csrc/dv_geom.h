@@ -153,4 +153,22 @@ DV_HD void projectCone(T& g0, T& g1, T& g2, T mu) {
   }
 }
 
+// Shape functions of the 4-node ANCF shell element (System.cu:904-961,
+// CollisionDetector.cu:50-69) at (xi, eta) in [0,1]^2 for sides a (along xi)
+// and b (along eta).  S[3q], S[3q+1], S[3q+2] weight node q's position,
+// x-gradient and y-gradient; node q = 0..3 sits at corner (0,0), (1,0),
+// (1,1), (0,1).  With u = |xi - xi_q| and w = |eta - eta_q| they are
+// (1-u)(1-w)(1+u+w-2u^2-2w^2), +-a u(1-u)^2 (1-w) and +-b w(1-w)^2 (1-u), the
+// sign negative where u or w runs against xi or eta.
+DV_HD void shellShape(double xi, double eta, double a, double b, double S[12]) {
+  for (int q = 0; q < 4; ++q) {
+    const bool fx = q == 1 || q == 2, fy = q >= 2;
+    const double u = fx ? 1.0 - xi : xi, w = fy ? 1.0 - eta : eta;
+    const double ou = 1.0 - u, ow = 1.0 - w;
+    S[3 * q] = (ou * ow) * (((1.0 + u) + w) - 2.0 * (u * u) - 2.0 * (w * w));
+    S[3 * q + 1] = (fx ? -a : a) * ((u * (ou * ou)) * ow);
+    S[3 * q + 2] = (fy ? -b : b) * ((w * (ow * ow)) * ou);
+  }
+}
+
 }  // namespace dv

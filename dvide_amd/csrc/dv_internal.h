@@ -180,6 +180,24 @@ struct Scene {
   std::vector<int> hBilA, hBilB;
   std::vector<double> hBilVel, hBilT0, hBilP0;
 
+  // --- fixed ANCF shell mesh (System::importMesh, System.cu:1932-2060).  Its
+  // contact points (System.cu:494-499) join the bodies at initialisation as
+  // fixed spheres of half the thickness in collision family -2, after the
+  // caller's bodies and in the reference's collision-geometry order, so
+  // collision detection, assembly and the solvers treat them as bodies; the
+  // boundary maps them back onto the shell's 9 DOFs per node.
+  int nUser = 0;                    // the caller's bodies (from initialisation)
+  int nNodes = 0;
+  std::vector<double> hShellNodes;  // 9 per node: position, x-gradient, y-gradient
+  std::vector<double> hShellFext;   // 9 per node
+  std::vector<int> hShellConn;      // 4 node indices per element
+  std::vector<double> hShellGeom;   // 3 per element: a, b, thickness
+  int shellPoints = 0;              // contact points per element side
+  std::vector<int> hPtEl;           // per contact point: its element
+  std::vector<double> hPtS;         // per contact point: 12 shape-function weights
+  DevBuf<double> shellNodes, ptS, shellF;
+  DevBuf<int> shellConn, ptEl, nodeIncStart, nodeInc;
+
   // --- device state
   hipStream_t stream = nullptr;
   int device = 0, numCU = 0;
@@ -240,6 +258,13 @@ void collide(Scene& s);
 // before the first wait
 void collideCount(Scene& s);
 void collideEmit(Scene& s);
+
+// fixed ANCF shells (dv_shell.hip): the contact points appended to the
+// host body arrays, their positions computed on the device, and the contact
+// force on the shell DOFs (9 per node, into a device buffer of 9 nNodes)
+void shellAppendPoints(Scene& s);
+void shellPlacePoints(Scene& s);
+void shellForces(Scene& s, double* out);
 
 // assembly of k and r, update (dv_assemble.hip)
 void buildK(Scene& s);

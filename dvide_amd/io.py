@@ -16,8 +16,10 @@
 Numbers are written as C++ streams write them by default (6 significant
 digits, "%g"), so files match the reference's byte for byte at the default
 precision; precision=17 writes every double exactly (checkpoint/resume).
-Only rigid bodies exist on this path: the beam, plate, shell and 2D-body
-counts are written as 0 and ignored on import.
+These formats carry rigid bodies only: the beam, plate, shell and 2D-body
+counts are written as 0 and ignored on import, and export_system and
+export_collisions refuse a scene with a shell mesh (export_matrices refuses
+it through k, which would need the shell's elastic forces).
 """
 from __future__ import annotations
 
@@ -122,7 +124,13 @@ def write_contact_forces(filename, idA, idB, loc, normal, friction, nb, precisio
 
 # --- System-level entry points ------------------------------------------------
 
+def _bodies_only(system, what):
+    if system.numShellNodes:
+        raise NotImplementedError(f"{what} writes rigid bodies only; this scene has a shell mesh")
+
+
 def export_system(system, filename, precision=6):
+    _bodies_only(system, "export_system")
     nb = system.numBodies
     write_system(filename, system.p.reshape(nb, 3), system.v.reshape(nb, 3), system.geometry().reshape(nb, 3),
                  system.fixed(), precision)
@@ -152,6 +160,7 @@ def export_matrices(system, directory):
 
 
 def export_collisions(system, filename, precision=6):
+    _bodies_only(system, "export_collisions")
     nb = system.numBodies
     a, b, nap, _ = system.contacts()
     write_collisions(filename, a, b, nap, system.p.reshape(nb, 3), system.v.reshape(nb, 3),

@@ -31,7 +31,7 @@ VEC_P, VEC_V, VEC_GAMMA, VEC_F_CONTACT, VEC_R, VEC_K, VEC_B = range(7)
 
 EXPORTED = (
     "dv_scene_create", "dv_scene_destroy", "dv_last_error", "dv_add_bodies", "dv_add_bilateral_dof",
-    "dv_set_bins_per_axis", "dv_set_param", "dv_get_param", "dv_initialize", "dv_step",
+    "dv_import_mesh", "dv_add_shell_mesh", "dv_num_shell_nodes", "dv_set_bins_per_axis", "dv_set_param", "dv_get_param", "dv_initialize", "dv_step",
     "dv_num_bodies", "dv_get_bodies", "dv_num_contacts", "dv_get_stats", "dv_get_vector", "dv_set_vector",
     "dv_get_contacts", "dv_ccp_violation", "dv_assemble", "dv_schur_product", "dv_schur_product_noalloc",
     "dv_pass_product", "dv_project_cone", "dv_solve", "dv_get_jacobian", "dv_contact_forces", "dv_energies",
@@ -107,6 +107,9 @@ def load_library(path: str = LIB_PATH):
         "dv_last_error": ([vp], ctypes.c_char_p),
         "dv_add_bodies": ([vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _ip], ctypes.c_int),
         "dv_add_bilateral_dof": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+        "dv_import_mesh": ([vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+        "dv_add_shell_mesh": ([vp, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp], ctypes.c_int),
+        "dv_num_shell_nodes": ([vp, _ip], ctypes.c_int),
         "dv_set_bins_per_axis": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "dv_set_param": ([vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "dv_get_param": ([vp, ctypes.c_int, _dp], ctypes.c_int),
@@ -342,6 +345,34 @@ class System:
         self._check(self._L.dv_add_bilateral_dof(self._h, int(dofA), int(dofB), float(velocity),
                                                  float(startTime)))
 
+    def importMesh(self, filename, stiffness, numContactPointsPerElement, pressure=0.0):
+        """System::importMesh (System.cu:1932-2060): a fixed ANCF shell mesh
+        from the reference's text format (a mesh with inverse mass raises
+        DvideError with DV_ERR_UNSUPPORTED)."""
+        self._flush_pending()
+        self._check(self._L.dv_import_mesh(self._h, os.fsencode(filename), float(stiffness),
+                                           int(numContactPointsPerElement), float(pressure)))
+
+    def add_shell_mesh(self, mesh):
+        """A fixed shell mesh from a scenes.ShellMesh (the arrays importMesh
+        fills), without a file."""
+        if len(mesh.inv_mass[2]) and np.any(np.asarray(mesh.inv_mass[2]) != 0):
+            raise DvideError(DV_ERR_UNSUPPORTED, "only fixed shells (no inverse mass) are supported")
+        self._flush_pending()
+        nodes = _f64(mesh.nodes)
+        conn = np.ascontiguousarray(mesh.conn, np.int32)
+        geom = _f64(mesh.geom)
+        fext = _f64(mesh.fext)
+        self._check(self._L.dv_add_shell_mesh(self._h, mesh.n_nodes, nodes.ctypes.data_as(_dp), mesh.n_elements,
+                                              conn.ctypes.data_as(_ip), geom.ctypes.data_as(_dp),
+                                              int(mesh.points), fext.ctypes.data_as(_dp)))
+
+    @property
+    def numShellNodes(self):
+        n = ctypes.c_int()
+        self._check(self._L.dv_num_shell_nodes(self._h, ctypes.byref(n)))
+        return n.value
+
     def initializeSystem(self):
         self._flush_pending()
         self._check(self._L.dv_initialize(self._h))
@@ -555,6 +586,8 @@ def from_scene(scene, build: int = BUILD_FP64, *, tol=None, max_iterations=None,
     if grid:
         s.setGrid(grid)
     s.add_bodies(scene.pos, scene.vel, scene.mass, scene.geom, scene.fixed, scene.family)
+    if getattr(scene, "shell", None) is not None:
+        s.add_shell_mesh(scene.shell)
     for (a, b), (vel, t0) in zip(scene.bil_dofs, scene.bil_info):
         s.addBilateralConstraintDOF(int(a), int(b), float(vel), float(t0))
     s.initializeSystem()

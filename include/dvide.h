@@ -153,6 +153,30 @@ int dv_add_bodies(dv_scene* s, int n, const double* pos, const double* vel, cons
  * from `start_time` on. */
 int dv_add_bilateral_dof(dv_scene* s, int dofA, int dofB, double velocity, double start_time);
 
+/* System::importMesh(filename, stiffness, numContactPointsPerElement,
+ * pressure), System.cu:1932-2060: an ANCF shell mesh in the reference's text
+ * format (a header line "nodes, shells, M entries, M^-1 entries", then the
+ * nodes' position and two gradient rows, connectivity, materials, geometry
+ * (a, b, thickness, -), the element-to-DOF map, external forces and the M and
+ * M^-1 triplets, 1-based).  Its DOFs follow the bodies' (9 per node) in every
+ * DOF vector and in D's columns.  Each element carries (points - 2)^2 contact
+ * points (System.cu:494-499), spheres of half the thickness in collision
+ * family -2, whose D blocks are 3x36 (System.cu:904-961).  Only fixed shells
+ * are supported: a mesh with a nonzero M^-1 entry (a shell free to move,
+ * which needs the shell's elastic forces) fails with DV_ERR_UNSUPPORTED, and
+ * stiffness and pressure, which act on free DOFs only, are not used.  One
+ * mesh per scene, after the bodies and before dv_initialize. */
+int dv_import_mesh(dv_scene* s, const char* path, double stiffness, int points, double pressure);
+
+/* The same fixed shell mesh from arrays (no counterpart in the reference;
+ * the arrays importMesh fills): nodes[9*n_nodes] (position, x-gradient,
+ * y-gradient per node), connectivity[4*n_elements] (0-based),
+ * geometry[4*n_elements] (a, b, thickness, ignored), `points` contact points
+ * per element side (as importMesh's numContactPointsPerElement), and fext[9*
+ * n_nodes] (may be NULL: zeros), which enters the potential energy only. */
+int dv_add_shell_mesh(dv_scene* s, int n_nodes, const double* nodes, int n_elements, const int* connectivity,
+                      const double* geometry, int points, const double* fext);
+
 /* CollisionDetector::setBinsPerAxis, CollisionDetector.cu:6 (accepted for API
  * parity; the spatial-hash broadphase sizes its cells from the geometry). */
 int dv_set_bins_per_axis(dv_scene* s, int bx, int by, int bz);
@@ -170,15 +194,22 @@ int dv_step(dv_scene* s);
 /* --- data access (System's public device vectors) ------------------------- */
 
 int dv_num_bodies(const dv_scene* s, int* n);
+/* nodes of the scene's shell mesh (System::nodes_h / 3); 9 DOFs each */
+int dv_num_shell_nodes(const dv_scene* s, int* n);
 /* System::bodies (Body::getMass, getGeometry, isFixed, getCollisionFamily):
  * copies of the first min(n, count) bodies' fields; any pointer may be NULL */
 int dv_get_bodies(const dv_scene* s, double* mass, double* geom, int* fixed, int* family, size_t n);
 int dv_num_contacts(const dv_scene* s, int* nc);
 int dv_get_stats(const dv_scene* s, dv_stats* out);
 
-/* copies min(n, length) entries of the vector into out; *len gets its length */
+/* copies min(n, length) entries of the vector into out; *len gets its length.
+ * DOF vectors (p, v, f_contact, k) hold the bodies' 3 DOFs each, then a shell
+ * mesh's 9 per node (fixed: its nodal coordinates, zero velocity, and the
+ * contact force D^T gamma / h on its DOFs); k of a scene with a shell fails
+ * with DV_ERR_UNSUPPORTED (it would need the shell's elastic forces). */
 int dv_get_vector(dv_scene* s, int which, double* out, size_t n, size_t* len);
-/* overwrites p or v (drivers mutate p_d/v_d between steps, models/test_draft.cu:438-439) */
+/* overwrites p or v (drivers mutate p_d/v_d between steps, models/test_draft.cu:438-439);
+ * a fixed shell's DOFs must keep their values */
 int dv_set_vector(dv_scene* s, int which, const double* in, size_t n);
 
 /* CollisionDetector::collisionIdentifierA_d/B_d and normalsAndPenetrations_d
@@ -194,9 +225,10 @@ int dv_ccp_violation(dv_scene* s, double out[4]);
 
 /* System::D (System.cu:1351-1386) as row-ordered COO: the bilateral rows
  * first (+1 at dofA, -1 at dofB), then per contact the rows n, u, v, each with
- * body A's three columns (+) before body B's (-).  *nnz, *rows, *cols get the
- * sizes; I, J, V (cap entries, may be NULL to query) get the entries.  Feeds
- * System::exportMatrices (System.cu:1912-1930). */
+ * body A's three columns (+) before body B's (-), or for a shell contact
+ * point its element's 36 columns weighted by the shape functions.  *nnz,
+ * *rows, *cols get the sizes; I, J, V (cap entries, may be NULL to query) get
+ * the entries.  Feeds System::exportMatrices (System.cu:1912-1930). */
 int dv_get_jacobian(dv_scene* s, int* I, int* J, double* V, size_t cap, size_t* nnz, int* rows, int* cols);
 
 /* System::calculateContactForcePerCollision (System.cu:1259-1289): per contact

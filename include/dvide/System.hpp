@@ -145,6 +145,18 @@ class System {
     bodies.push_back(b);
     return (int)bodies.size();
   }
+  // System::importMesh (System.cu:1932-2060): an ANCF shell mesh file, read
+  // at initializeSystem after the bodies (its DOFs follow theirs).  Fixed
+  // shells only: a mesh with inverse mass makes initializeSystem throw
+  // dvide::Error with DV_ERR_UNSUPPORTED.
+  void importMesh(const std::string& filename, double stiffness, int numContactPointsPerElement,
+                  double pressure = 0.0) {
+    meshFile_ = filename;
+    meshStiffness_ = stiffness;
+    meshPoints_ = numContactPointsPerElement;
+    meshPressure_ = pressure;
+  }
+
   int addBilateralConstraintDOF(int a, int b) { return addBilateralConstraintDOF(a, b, 0.0, 0.0); }
   int addBilateralConstraintDOF(int a, int b, double velocity, double startTime) {
     check(dv_add_bilateral_dof(s_, a, b, velocity, startTime));
@@ -166,6 +178,7 @@ class System {
     }
     if (n) check(dv_add_bodies(s_, (int)n, pos.data(), vel.data(), mass.data(), geom.data(), fixed.data(),
                                family.data()));
+    if (!meshFile_.empty()) check(dv_import_mesh(s_, meshFile_.c_str(), meshStiffness_, meshPoints_, meshPressure_));
     // gravity enters the applied forces once, here (System.cu:344-424)
     check(dv_set_param(s_, DV_PARAM_GX, gravity.x));
     check(dv_set_param(s_, DV_PARAM_GY, gravity.y));
@@ -342,6 +355,9 @@ class System {
 
   dv_scene* s_ = nullptr;
   std::vector<std::unique_ptr<Body>> owned_;  // bodies created by importSystem
+  std::string meshFile_;                      // importMesh's arguments
+  double meshStiffness_ = 0, meshPressure_ = 0;
+  int meshPoints_ = 0;
   APGD apgd_;
   CollisionDetector cd_;
 };

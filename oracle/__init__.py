@@ -66,6 +66,11 @@ def lib():
             "or_frame": ([_dp, _dp, _dp], None),
             "or_narrow_pair": ([vp, ctypes.c_int, ctypes.c_int, _dp, _dp], ctypes.c_int),
             "or_ccp_violation": ([vp, _dp], None),
+            "or_add_shell": ([vp, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, ctypes.c_long, _ip, _ip, _dp,
+                              ctypes.c_long, _ip, _ip, _dp], ctypes.c_int),
+            "or_num_geometries": ([vp], ctypes.c_int),
+            "or_shell_point": ([vp, ctypes.c_int, _dp, _dp], None),
+            "or_shell_shape": ([ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -112,6 +117,20 @@ class OracleSystem:
         L.or_add_bodies(self._h, scene.n, pos.ctypes.data_as(_dp), vel.ctypes.data_as(_dp),
                         mass.ctypes.data_as(_dp), geom.ctypes.data_as(_dp),
                         fixed.ctypes.data_as(_ip), fam.ctypes.data_as(_ip))
+        mesh = getattr(scene, "shell", None)
+        if mesh is not None:
+            self._keep = [np.ascontiguousarray(x, dt) for x, dt in (
+                (mesh.nodes, np.float64), (mesh.conn, np.int32), (mesh.geom, np.float64), (mesh.fext, np.float64),
+                (mesh.mass[0], np.int32), (mesh.mass[1], np.int32), (mesh.mass[2], np.float64),
+                (mesh.inv_mass[0], np.int32), (mesh.inv_mass[1], np.int32), (mesh.inv_mass[2], np.float64))]
+            geom = self._keep[2].copy()
+            geom[:, 3] = mesh.points  # importMesh's numContactPointsPerElement
+            self._keep[2] = geom
+            k = self._keep
+            L.or_add_shell(self._h, mesh.n_nodes, k[0].ctypes.data_as(_dp), mesh.n_elements,
+                           k[1].ctypes.data_as(_ip), k[2].ctypes.data_as(_dp), k[3].ctypes.data_as(_dp),
+                           len(k[6]), k[4].ctypes.data_as(_ip), k[5].ctypes.data_as(_ip), k[6].ctypes.data_as(_dp),
+                           len(k[9]), k[7].ctypes.data_as(_ip), k[8].ctypes.data_as(_ip), k[9].ctypes.data_as(_dp))
         for (a_, b_), (vv, t0) in zip(scene.bil_dofs, scene.bil_info):
             L.or_add_bilateral(self._h, int(a_), int(b_), float(vv), float(t0))
         L.or_initialize(self._h)
@@ -209,6 +228,25 @@ class OracleSystem:
         out = np.zeros(4)
         self._L.or_ccp_violation(self._h, out.ctypes.data_as(_dp))
         return out
+
+    @property
+    def num_geometries(self):
+        """Collision geometries: one per body, then the shells' contact points."""
+        return self._L.or_num_geometries(self._h)
+
+    def shell_point(self, j):
+        """Position and shape-function weights of shell contact point j."""
+        pos, S = np.zeros(3), np.zeros(12)
+        self._L.or_shell_point(self._h, int(j), pos.ctypes.data_as(_dp), S.ctypes.data_as(_dp))
+        return pos, S
+
+
+def shell_shape(xi, eta, a, b):
+    """The 12 shape-function weights of the ANCF shell element at (xi, eta):
+    per corner node, its position, x-gradient and y-gradient weights."""
+    S = np.zeros(12)
+    lib().or_shell_shape(float(xi), float(eta), float(a), float(b), S.ctypes.data_as(_dp))
+    return S
 
 
 def project3(g, mu):

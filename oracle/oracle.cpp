@@ -16,6 +16,9 @@
 //   PGJ / PGS (secondary cross-check solvers)      PGJ.cu:66-81, 151-210; PGS.cu:149-252, 384-447
 //   DoTimeStep update                              System.cu:544-602
 //   getCCPViolation                                System.cu:1590-1643
+//   ANCF shell meshes: importMesh, contact points,
+//     their positions and 3x36 Jacobian blocks     System.cu:474-500, 676-767, 904-967, 1932-2060;
+//                                                  CollisionDetector.cu:50-69, 226-280
 //
 // Linear algebra mirrors CUSP semantics: D and D^T are COO matrices sorted by
 // row (D^T by a stable sort, as cusp::coo_matrix::sort_by_row), and a COO
@@ -32,6 +35,10 @@
 // the stick/slip threshold |g_x| = mu g, the pile weight balance, the CCP
 // optimality conditions at convergence, and the velocities of converged
 // projected Gauss-Seidel and Jacobi runs (another route to the same CCP).
+// The shell element's shape functions are pinned by their corner values and
+// slopes, the exact interpolation of a flat element, and the reference's
+// expanded polynomials; the shell Jacobian blocks by rigid translations and
+// by the dense product D q against the contact points' positions.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -153,6 +160,25 @@ struct Aabb {
   V3 lo, hi;
 };
 
+// Shape functions of the 4-node ANCF shell element (the incomplete cubic
+// interpolation of System.cu:904-961 and CollisionDetector.cu:50-69) at
+// (xi, eta) in [0,1]^2, for an element of side lengths a (along xi) and b
+// (along eta).  Node q sits at corner (0,0), (1,0), (1,1), (0,1) for q = 0..3;
+// S[3q] weights its position, S[3q+1] its x-gradient and S[3q+2] its
+// y-gradient.  In the corner's own coordinates u = |xi - xi_q|, w = |eta -
+// eta_q|: position (1-u)(1-w)(1+u+w-2u^2-2w^2), x-gradient +-a u(1-u)^2 (1-w),
+// y-gradient +-b w(1-w)^2 (1-u), the sign following the direction of u, w.
+void shellShape(double xi, double eta, double a, double b, double S[12]) {
+  for (int q = 0; q < 4; ++q) {
+    const bool fx = q == 1 || q == 2, fy = q >= 2;
+    const double u = fx ? 1.0 - xi : xi, w = fy ? 1.0 - eta : eta;
+    const double ou = 1.0 - u, ow = 1.0 - w;
+    S[3 * q] = (ou * ow) * (((1.0 + u) + w) - 2.0 * (u * u) - 2.0 * (w * w));
+    S[3 * q + 1] = (fx ? -a : a) * ((u * (ou * ou)) * ow);
+    S[3 * q + 2] = (fy ? -b : b) * ((w * (ow * ow)) * ou);
+  }
+}
+
 struct System {
   // parameters (System.cu:14-28, CollisionDetector.cu:503-514, APGD.cu:6-16)
   double h = 1e-3, mu = 0.25, tol = 1e-4, envelope = 1e-3;
@@ -172,10 +198,26 @@ struct System {
 
   // bodies
   int nb = 0;
-  std::vector<double> p, v, f, fEl, k, fContact, tmp, minv;  // minv per DOF
+  std::vector<double> p, v, f, fEl, k, fContact, tmp, minv;  // minv per body DOF
   std::vector<double> geom;                                  // 3 per body
   std::vector<int> fixed, family;
   bool initialized = false;
+
+  // ANCF shell mesh (System::importMesh, System.cu:1932-2060).  Its DOFs
+  // follow the bodies': 9 per node (position, x-gradient, y-gradient).
+  int nNodes = 0;
+  std::vector<double> shellNodes;  // 9 per node
+  std::vector<int> shellConn;      // 4 node indices per element
+  std::vector<double> shellGeom;   // 4 per element: a, b, thickness, points per side
+  std::vector<double> shellFext;   // 9 per node
+  Coo shellMass;                   // M of the shell DOFs (rows and columns from 0)
+  std::vector<int> shellInvI, shellInvJ;  // M^-1 triplets of the shell DOFs
+  std::vector<double> shellInvV;
+  // collision geometries: one per body, then each element's contact points
+  // (System.cu:494-499: i, k = 1 .. points-2, xi = i/(points-1), eta = k/(points-1))
+  int ng = 0;
+  std::vector<int> ptEl, ptI, ptK;
+  int ndof() const { return 3 * nb + 9 * nNodes; }
 
   // bilateral DOF constraints (System.cu:510-532)
   std::vector<int> bilA, bilB, bilOffset;
@@ -203,26 +245,64 @@ struct System {
   // --- System::initializeSystem (bodies) and processConstraints -------------
   void initialize() {
     nb = (int)fixed.size();
-    const int n3 = 3 * nb;
-    f.assign(n3, 0.0);
-    fEl.assign(n3, 0.0);
-    k.assign(n3, 0.0);
-    fContact.assign(n3, 0.0);
-    tmp.assign(n3, 0.0);
-    M.rows = M.cols = n3;
-    M.I.resize(n3); M.J.resize(n3); M.V.resize(n3);
+    const int n3 = 3 * nb, nd = ndof();
+    // the shell DOFs follow the bodies' in p, v, f (System.cu:440-466)
+    p.resize(n3);
+    v.resize(n3);
+    p.insert(p.end(), shellNodes.begin(), shellNodes.end());
+    v.resize(nd, 0.0);
+    f.assign(nd, 0.0);
+    for (int i = 0; i < 9 * nNodes; ++i) f[n3 + i] = shellFext[i];
+    fEl.assign(nd, 0.0);  // the shells' elastic forces are not restated: zero
+    k.assign(nd, 0.0);
+    fContact.assign(nd, 0.0);
+    tmp.assign(nd, 0.0);
     minv.resize(n3);
+    // M^-1: the bodies' diagonal, then the shell triplets at offset 3 nb
+    // (System.cu:466-472), sorted by row
+    std::vector<int> I, J;
+    std::vector<double> V;
     for (int j = 0; j < nb; ++j) {
       for (int c = 0; c < 3; ++c) {
         double g = c == 0 ? gravity.x : (c == 1 ? gravity.y : gravity.z);
         f[3 * j + c] = fixed[j] ? 0.0 : bodyMass[j] * g;
-        M.I[3 * j + c] = M.J[3 * j + c] = 3 * j + c;
-        M.V[3 * j + c] = fixed[j] ? 0.0 : 1.0 / bodyMass[j];
-        minv[3 * j + c] = M.V[3 * j + c];
+        I.push_back(3 * j + c);
+        J.push_back(3 * j + c);
+        V.push_back(fixed[j] ? 0.0 : 1.0 / bodyMass[j]);
+        minv[3 * j + c] = V.back();
       }
     }
+    for (size_t q = 0; q < shellInvI.size(); ++q) {
+      I.push_back(n3 + shellInvI[q]);
+      J.push_back(n3 + shellInvJ[q]);
+      V.push_back(shellInvV[q]);
+    }
+    std::vector<size_t> perm(I.size());
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t b) { return I[a] < I[b]; });
+    M = Coo();
+    M.rows = M.cols = nd;
+    for (size_t q : perm) {
+      M.I.push_back(I[q]);
+      M.J.push_back(J[q]);
+      M.V.push_back(V[q]);
+    }
     M.index();
-    gamma.assign(n3, 0.0);  // gamma_d = a_h (System.cu:185): zeros, one per DOF
+    shellMass.rows = shellMass.cols = 9 * nNodes;
+    shellMass.index();
+    // collision geometries: the bodies', then the shells' contact points
+    ptEl.clear(); ptI.clear(); ptK.clear();
+    for (size_t e = 0; e < shellConn.size() / 4; ++e) {
+      const int w = (int)shellGeom[4 * e + 3];
+      for (int i = 1; i < w - 1; ++i)
+        for (int kk = 1; kk < w - 1; ++kk) {
+          ptEl.push_back((int)e);
+          ptI.push_back(i);
+          ptK.push_back(kk);
+        }
+    }
+    ng = nb + (int)ptEl.size();
+    gamma.assign(nd, 0.0);  // gamma_d = a_h (System.cu:185): zeros, one per DOF
     bilOffset.clear();
     bilP0.assign(bilA.size(), 0.0);
     int off = 0;
@@ -241,18 +321,46 @@ struct System {
   std::vector<double> bodyMass;
 
   V3 bodyPos(int g) const { return mk(p[3 * g], p[3 * g + 1], p[3 * g + 2]); }
+
+  // contact point j of the shells: its element's shape functions at (xi, eta)
+  void pointShape(int j, double S[12]) const {
+    const int e = ptEl[j];
+    const double w1 = shellGeom[4 * e + 3] - 1.0;
+    shellShape((double)ptI[j] / w1, (double)ptK[j] / w1, shellGeom[4 * e], shellGeom[4 * e + 1], S);
+  }
+  // its position from the element's nodal coordinates in p
+  // (CollisionDetector.cu:50-69), summed node by node
+  V3 pointPos(int j) const {
+    double S[12];
+    pointShape(j, S);
+    const int e = ptEl[j];
+    double x[3] = {0, 0, 0};
+    for (int q = 0; q < 4; ++q) {
+      const double* nd = &p[3 * (size_t)nb + 9 * (size_t)shellConn[4 * e + q]];
+      for (int c = 0; c < 3; ++c) x[c] += (S[3 * q] * nd[c] + S[3 * q + 1] * nd[3 + c]) + S[3 * q + 2] * nd[6 + c];
+    }
+    return mk(x[0], x[1], x[2]);
+  }
+  // collision geometry g: a body's, or a shell contact point (a sphere of
+  // half the thickness, collision family -2, System.cu:494-499)
+  V3 geomPos(int g) const { return g < nb ? bodyPos(g) : pointPos(g - nb); }
+  V3 geomVec(int g) const {
+    if (g < nb) return mk(geom[3 * g], geom[3 * g + 1], geom[3 * g + 2]);
+    return mk(0.5 * shellGeom[4 * ptEl[g - nb] + 2], 0, 0);
+  }
+  int famOf(int g) const { return g < nb ? family[g] : -2; }
+  int ownerOf(int g) const { return g < nb ? g : nb + ptEl[g - nb]; }  // collisionMap.x
   V3 halfExt(int g) const {
-    V3 e = mk(geom[3 * g], geom[3 * g + 1], geom[3 * g + 2]);
+    V3 e = geomVec(g);
     if (e.y == 0) e = mk(e.x, e.x, e.x);
     return e;
   }
-  bool isSphere(int g) const { return geom[3 * g + 1] == 0; }
 
   // --- generateAabbData (CollisionDetector.cu:24-79) ------------------------
   std::vector<Aabb> aabbs() const {
-    std::vector<Aabb> a(nb);
-    for (int g = 0; g < nb; ++g) {
-      V3 pos = bodyPos(g), e = halfExt(g);
+    std::vector<Aabb> a(ng);
+    for (int g = 0; g < ng; ++g) {
+      V3 pos = geomPos(g), e = halfExt(g);
       e = e + mk(envelope, envelope, envelope);
       a[g].lo = pos - e;
       a[g].hi = pos + e;
@@ -264,7 +372,7 @@ struct System {
   std::vector<uint64_t> broadphase() {
     std::vector<Aabb> a = aabbs();
     std::vector<uint64_t> keys;
-    if (nb == 0) return keys;
+    if (ng == 0) return keys;
     V3 lo = a[0].lo, hi = a[0].lo;
     for (const Aabb& b : a) {
       for (V3 q : {b.lo, b.hi}) {
@@ -282,7 +390,7 @@ struct System {
     }
     // (bin, aabb) pairs, then group by bin
     std::vector<std::pair<uint32_t, uint32_t>> bi;
-    for (int g = 0; g < nb; ++g) {
+    for (int g = 0; g < ng; ++g) {
       unsigned x0 = (unsigned)(a[g].lo.x * bsi.x), y0 = (unsigned)(a[g].lo.y * bsi.y),
                z0 = (unsigned)(a[g].lo.z * bsi.z);
       unsigned x1 = (unsigned)(a[g].hi.x * bsi.x), y1 = (unsigned)(a[g].hi.y * bsi.y),
@@ -308,8 +416,8 @@ struct System {
           bool in = (A.lo.x <= B.hi.x && B.lo.x <= A.hi.x) &&
                     (A.lo.y <= B.hi.y && B.lo.y <= A.hi.y) &&
                     (A.lo.z <= B.hi.z && B.lo.z <= A.hi.z);
-          if (ga == gb) in = false;  // same body (collisionMap.x equal)
-          if (family[ga] == family[gb] && family[ga] != -1) in = false;
+          if (ownerOf(ga) == ownerOf(gb)) in = false;  // same item (collisionMap.x equal)
+          if (famOf(ga) == famOf(gb) && famOf(ga) != -1) in = false;
           if (in) {
             uint64_t lo32 = std::min(ga, gb), hi32 = std::max(ga, gb);
             keys.push_back((lo32 << 32) | hi32);
@@ -327,9 +435,8 @@ struct System {
   //     (countActualCollisions CollisionDetector.cu:284-328,
   //      storeActualCollisions CollisionDetector.cu:432-498)
   bool narrow(int ga, int gb, double* nOut, double* pen, V3* locOut) const {
-    V3 posA = bodyPos(ga), posB = bodyPos(gb);
-    V3 gA = mk(geom[3 * ga], geom[3 * ga + 1], geom[3 * ga + 2]);
-    V3 gB = mk(geom[3 * gb], geom[3 * gb + 1], geom[3 * gb + 2]);
+    V3 posA = geomPos(ga), posB = geomPos(gb);
+    V3 gA = geomVec(ga), gB = geomVec(gb);
     V3 normal = mk(1, 0, 0), location = mk(0, 0, 0);
     double penetration = 0;
     if (gA.y == 0 && gB.y == 0) {
@@ -393,13 +500,18 @@ struct System {
 
   // --- buildAppliedImpulseVector (System.cu:1502-1514, 1411-1417) ------------
   void buildK() {
-    const int n3 = 3 * nb;
+    const int n3 = 3 * nb, nd = ndof();
     for (int i = 0; i < n3; ++i) {
       double m = minv[i];
       if (m != 0) m = 1.0 / m;
       k[i] = m * v[i];
     }
-    for (int i = 0; i < n3; ++i) k[i] = (h * f[i] + (-h) * fEl[i]) + 1.0 * k[i];
+    if (nNodes) {  // the shells' M v (System.cu:1510)
+      std::vector<double> vs(v.begin() + n3, v.end()), ks;
+      shellMass.mul(vs, ks);
+      for (int i = 0; i < 9 * nNodes; ++i) k[n3 + i] = ks[i];
+    }
+    for (int i = 0; i < nd; ++i) k[i] = (h * f[i] + (-h) * fEl[i]) + 1.0 * k[i];
   }
 
   // --- contact frame (System.cu:753-762) -------------------------------------
@@ -414,12 +526,36 @@ struct System {
   }
 
   // --- buildContactJacobian + transpose (System.cu:622-638, 676-1257, 1351-1409)
+  // columns and weights of one side of a contact row: a body's 3 DOFs, or
+  // the 36 DOFs of a shell point's element, node by node (position,
+  // x-gradient, y-gradient, each x, y, z), weighted by the shape functions
+  // (System.cu:785-800, 904-961)
+  int sideEntries(int g, int* cols, double* wts) const {
+    if (g < nb) {
+      for (int q = 0; q < 3; ++q) { cols[q] = 3 * g + q; wts[q] = 1.0; }
+      return 3;
+    }
+    double S[12];
+    pointShape(g - nb, S);
+    const int e = ptEl[g - nb];
+    for (int q = 0; q < 4; ++q)
+      for (int m = 0; m < 9; ++m) {
+        cols[9 * q + m] = 3 * nb + 9 * shellConn[4 * e + q] + m;
+        wts[9 * q + m] = S[3 * q + m / 3];
+      }
+    return 36;
+  }
+  int sideWidth(int g) const { return g < nb ? 3 : 36; }
+
   void buildD() {
     const int nbil = (int)bilA.size();
     D = Coo();
     D.rows = 3 * nc + nbil;
-    D.cols = 3 * nb;
-    const size_t nnz = (size_t)bilNnz + 18 * (size_t)nc;
+    D.cols = ndof();
+    // each contact row holds its A side's entries, then its B side's
+    std::vector<size_t> off(nc + 1, (size_t)bilNnz);
+    for (int c = 0; c < nc; ++c) off[c + 1] = off[c] + 3 * (size_t)(sideWidth(idA[c]) + sideWidth(idB[c]));
+    const size_t nnz = off[nc];
     D.I.resize(nnz); D.J.resize(nnz); D.V.resize(nnz);
     for (int i = 0; i < nbil; ++i) {
       int o = bilOffset[i];
@@ -429,20 +565,22 @@ struct System {
     friction.assign(nc, mu);
 #pragma omp parallel for schedule(static)
     for (long c = 0; c < nc; ++c) {
-      size_t o = (size_t)bilNnz + 18 * (size_t)c;
+      size_t o = off[c];
       V3 u, w;
       frame(&nAndP[4 * c], &u, &w);
       V3 n = mk(nAndP[4 * c], nAndP[4 * c + 1], nAndP[4 * c + 2]);
       const V3 rowsv[3] = {n, u, w};
-      int jA = 3 * (int)idA[c], jB = 3 * (int)idB[c];
+      int colA[36], colB[36];
+      double wA[36], wB[36];
+      const int na = sideEntries((int)idA[c], colA, wA), nbs = sideEntries((int)idB[c], colB, wB);
       for (int rr = 0; rr < 3; ++rr) {
         int row = 3 * (int)c + rr + nbil;
         const double vals[3] = {rowsv[rr].x, rowsv[rr].y, rowsv[rr].z};
-        for (int q = 0; q < 3; ++q) {
-          D.I[o] = row; D.J[o] = jA + q; D.V[o] = vals[q]; ++o;
+        for (int q = 0; q < na; ++q) {
+          D.I[o] = row; D.J[o] = colA[q]; D.V[o] = na == 3 ? vals[q] : wA[q] * vals[q % 3]; ++o;
         }
-        for (int q = 0; q < 3; ++q) {
-          D.I[o] = row; D.J[o] = jB + q; D.V[o] = -vals[q]; ++o;
+        for (int q = 0; q < nbs; ++q) {
+          D.I[o] = row; D.J[o] = colB[q]; D.V[o] = nbs == 3 ? -vals[q] : -(wB[q] * vals[q % 3]); ++o;
         }
       }
     }
@@ -605,7 +743,7 @@ struct System {
         // APGD.cu:335-345
         for (size_t i = 0; i < R; ++i) r[i] = -1.0 * antiRelaxVec[i] + r[i];
         DT.mul(gamma, fContact);
-        for (int i = 0; i < 3 * nb; ++i) tmp[i] = k[i] + fContact[i];
+        for (int i = 0; i < ndof(); ++i) tmp[i] = k[i] + fContact[i];
         M.mul(tmp, v);
         D.mul(v, t);
         const int off = (int)bilA.size();
@@ -676,12 +814,12 @@ struct System {
   void projectedIterations() {
     const double omega = 0.3, lambda = 2.0 / 3.0;  // PGS.cu:13-14, PGJ.cu:14-15
     iterations = -1;
-    if (!bilA.empty()) return;  // contact rows only
+    if (!bilA.empty() || nNodes) return;  // contact rows between bodies only
     gamma.assign(3 * (size_t)nc, 0.0);
     for (int c = 0; c < nc; ++c) gamma[3 * c] = 1.0;
     std::vector<double> B, w, gh;
     pgScale(B);
-    std::vector<double> vel(3 * (size_t)nb);
+    std::vector<double> vel((size_t)ndof());
     if (solverKind == 6) {  // v = M^-1 (D^T gamma) + M^-1 k
       DT.mul(gamma, fContact);
       M.mul(fContact, vel);
@@ -743,16 +881,16 @@ struct System {
       if (solverKind == 5 || solverKind == 6) projectedIterations();
       else apgd();
       DT.mul(gamma, fContact);
-      for (int i = 0; i < 3 * nb; ++i) tmp[i] = 1.0 * k[i] + 1.0 * fContact[i];
+      for (int i = 0; i < ndof(); ++i) tmp[i] = 1.0 * k[i] + 1.0 * fContact[i];
       M.mul(tmp, v);
-      for (int i = 0; i < 3 * nb; ++i) fContact[i] = (1.0 / h) * fContact[i];
+      for (int i = 0; i < ndof(); ++i) fContact[i] = (1.0 / h) * fContact[i];
     } else {
       M.mul(k, v);
       std::fill(fContact.begin(), fContact.end(), 0.0);
       iterations = 0;
       residual = 0;
     }
-    for (int i = 0; i < 3 * nb; ++i) p[i] = h * v[i] + p[i];
+    for (int i = 0; i < ndof(); ++i) p[i] = h * v[i] + p[i];
     time += h;
     timeIndex++;
   }
@@ -851,6 +989,48 @@ int or_add_bodies(void* sp, int n, const double* pos, const double* vel, const d
   }
   return (int)s->fixed.size();
 }
+
+// An ANCF shell mesh as System::importMesh reads it (System.cu:1932-2060):
+// nodes (9 per node), connectivity (4 per element), geometry (a, b,
+// thickness, contact points per side, 4 per element), external forces (9
+// per node), and the M and M^-1 triplets of the shell DOFs (0-based).  Before
+// or_initialize, after every body.
+int or_add_shell(void* sp, int nNodes, const double* nodes, int nEl, const int* conn, const double* geom,
+                 const double* fext, long nM, const int* mI, const int* mJ, const double* mV, long nInv,
+                 const int* iI, const int* iJ, const double* iV) {
+  System* s = static_cast<System*>(sp);
+  if (s->nNodes) return -1;
+  s->nNodes = nNodes;
+  s->shellNodes.assign(nodes, nodes + 9 * (size_t)nNodes);
+  s->shellConn.assign(conn, conn + 4 * (size_t)nEl);
+  s->shellGeom.assign(geom, geom + 4 * (size_t)nEl);
+  s->shellFext.assign(fext, fext + 9 * (size_t)nNodes);
+  std::vector<size_t> perm((size_t)nM);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t b) { return mI[a] < mI[b]; });
+  s->shellMass = Coo();
+  for (size_t q : perm) {
+    s->shellMass.I.push_back(mI[q]);
+    s->shellMass.J.push_back(mJ[q]);
+    s->shellMass.V.push_back(mV[q]);
+  }
+  s->shellInvI.assign(iI, iI + nInv);
+  s->shellInvJ.assign(iJ, iJ + nInv);
+  s->shellInvV.assign(iV, iV + nInv);
+  return 0;
+}
+
+int or_num_geometries(void* sp) { return static_cast<System*>(sp)->ng; }
+
+// Position and shape functions of shell contact point j (geometry nb + j).
+void or_shell_point(void* sp, int j, double* pos, double* S) {
+  System* s = static_cast<System*>(sp);
+  V3 x = s->pointPos(j);
+  pos[0] = x.x; pos[1] = x.y; pos[2] = x.z;
+  if (S) s->pointShape(j, S);
+}
+
+void or_shell_shape(double xi, double eta, double a, double b, double* S) { shellShape(xi, eta, a, b, S); }
 
 int or_add_bilateral(void* sp, int dofA, int dofB, double vel, double t0) {
   System* s = static_cast<System*>(sp);
