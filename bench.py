@@ -197,6 +197,44 @@ def lid_pressed(steps=5):
     return out
 
 
+def shell_bed_leg(settle=400, steps=20):
+    """Spheres on a fixed ANCF shell: 19,200 spheres r ~ U[0.02, 0.03] in
+    eight jittered lattice layers over a fixed 20 x 80 mesh of 0.1 m elements
+    (1,701 nodes; 8 contact points per element side, 57,600 points), mu 0.4,
+    h 5e-4, APGD tol 1e-4, warm start.  `settle` untimed steps let the layers
+    land, then `steps` timed steps per graph-driven solver: time-steps/s,
+    contacts, iterations and residuals, and the vertical contact force the
+    shell's nodes carry as a share of the spheres' weight."""
+    from dvide_amd import scenes, system
+
+    sc = scenes.shell_bed(3003, nx=20, ny=80, per_side=(24, 100), layers=8, tol=TOL)
+    weight = float(sc.mass.sum()) * 9.81
+    w = sc.shell.points - 2
+    out = {"spheres": sc.n, "shell_nodes": sc.shell.n_nodes, "contact_points": sc.shell.n_elements * w * w,
+           "settle_steps": settle}
+    for name, build in (("mixed", system.BUILD_MIXED), ("fp64", system.BUILD_FP64)):
+        g = system.from_scene(sc, build, tol=TOL, max_iterations=REFERENCE_CAP)
+        for _ in range(settle):
+            g.step()
+        nc, its, res = [], [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.step()
+            st = g.stats()
+            nc.append(st.num_contacts)
+            its.append(st.iterations)
+            res.append(st.residual)
+        el = time.perf_counter() - t0
+        fc = g.f_contact
+        out[name] = {"time_steps_per_s": round(steps / el, 2), "steps": steps, "contacts_mean": int(np.mean(nc)),
+                     "iterations_mean": round(float(np.mean(its)), 1), "iterations_max": int(np.max(its)),
+                     "residual_max": float(np.max(res)), "steps_converged": int(sum(r < TOL for r in res)),
+                     "shell_carries_weight_share": round(float(-fc[3 * sc.n:].reshape(-1, 9)[:, 1].sum()) / weight,
+                                                         4)}
+        g.close()
+    return out
+
+
 def _gen_many(specs):
     """Scenes generated on the host cores in parallel (the generator releases
     the interpreter lock)."""
@@ -545,6 +583,7 @@ def main():
     ap.add_argument("--only-mixed", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-scene batched solve")
     ap.add_argument("--no-serve", action="store_true", help="skip the served request stream")
+    ap.add_argument("--no-shell", action="store_true", help="skip the spheres-on-a-fixed-shell leg")
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the batched leg (with --batch-scenes scenes) and print its figures")
     ap.add_argument("--batch-scenes", type=int, default=512)
@@ -705,6 +744,9 @@ def main():
     if not args.only_mixed:
         progress("lid-pressed leg")
         line["lid_pressed"] = lid_pressed()
+    if not args.only_mixed and not args.no_shell:
+        progress("shell-bed leg")
+        line["shell_bed"] = shell_bed_leg()
     if not args.no_batch and not args.only_mixed:
         progress(f"{args.batch_scenes}-scene batch")
         line["batched"] = batched(n_scenes=args.batch_scenes)
