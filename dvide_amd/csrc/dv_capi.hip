@@ -385,11 +385,10 @@ int dv_add_shell_mesh(dv_scene* h, int n_nodes, const double* nodes, int n_eleme
   });
 }
 
-int dv_import_mesh(dv_scene* h, const char* path, double stiffness, int points_per_side, double pressure) {
-  (void)stiffness;  // the elastic and pressure forces act on free shell DOFs only
-  (void)pressure;
-  if (!h) return DV_ERR_ARG;
-  Scene& s = h->s;
+// System::importMesh's reader (System.cu:1932-2060): one record per line,
+// fields separated by commas or blanks; the arrays it fills go to
+// dv_add_shell_mesh
+static int importMesh(dv_scene* h, Scene& s, const char* path, int points_per_side) {
   if (!path) return dv::fail(s, DV_ERR_ARG, "null path");
   std::ifstream in(path);
   if (!in) return dv::fail(s, DV_ERR_ARG, (std::string("cannot open ") + path).c_str());
@@ -406,6 +405,7 @@ int dv_import_mesh(dv_scene* h, const char* path, double stiffness, int points_p
   long nn = 0, ne = 0, nM = 0, nInv = 0;
   if (!next(ss) || !(ss >> nn >> ne >> nM >> nInv) || nn <= 0 || ne <= 0 || nM < 0 || nInv < 0)
     return dv::fail(s, DV_ERR_ARG, "bad mesh header (nodes, shells, M entries, M^-1 entries)");
+  if (nn > (1L << 24) || ne > (1L << 24)) return dv::fail(s, DV_ERR_ARG, "mesh larger than 2^24 nodes or shells");
   std::vector<double> nodes(9 * nn), fext(9 * nn), geom(4 * ne);
   std::vector<int> conn(4 * ne);
   for (long i = 0; i < 3 * nn; ++i)
@@ -443,6 +443,12 @@ int dv_import_mesh(dv_scene* h, const char* path, double stiffness, int points_p
   }
   return dv_add_shell_mesh(h, (int)nn, nodes.data(), (int)ne, conn.data(), geom.data(), points_per_side,
                            fext.data());
+}
+
+int dv_import_mesh(dv_scene* h, const char* path, double stiffness, int points_per_side, double pressure) {
+  (void)stiffness;  // the elastic and pressure forces act on free shell DOFs only
+  (void)pressure;
+  return dv::guarded(h, [&](Scene& s) { return importMesh(h, s, path, points_per_side); });
 }
 
 int dv_set_bins_per_axis(dv_scene* h, int bx, int by, int bz) {

@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(_HERE, "libdvide_hip.so")
 
 # include/dvide.h
 DV_OK = 0
-DV_ERR_UNSUPPORTED = -5
+DV_ERR_ARG, DV_ERR_STATE, DV_ERR_HIP, DV_ERR_NOMEM, DV_ERR_UNSUPPORTED, DV_ERR_DIVERGED = -1, -2, -3, -4, -5, -6
 SOLVER_APGD = 1  # System(int solverType): 2..7 (PDIP .. PJKIP) are rejected
 BUILD_FP64, BUILD_MIXED, BUILD_PLAIN = 0, 1, 2  # DV_APGD_*
 (PARAM_H, PARAM_MU, PARAM_TOL, PARAM_MAX_ITER, PARAM_GX, PARAM_GY, PARAM_GZ, PARAM_ENVELOPE,

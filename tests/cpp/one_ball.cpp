@@ -3,8 +3,10 @@
 // normal impulse, contact force and CCP violation printed for the test to
 // check (gamma_n = m g h, f_contact = m g); then System::exportSystem,
 // System::exportMatrices into argv[2], a second System built by
-// System::importSystem from the exported file and stepped once, and the
-// factory's answer to a solver type that is not on this path (2 = PDIP).
+// System::importSystem from the exported file and stepped once, the
+// factory's answer to a solver type that is not on this path (2 = PDIP),
+// and, given a mesh file and a position, a ball resting on a fixed shell
+// read by System::importMesh.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -63,6 +65,32 @@ int main(int argc, char** argv) {
   } catch (const dvide::Error& e) {
     std::fprintf(stderr, "error %d: %s\n", e.code, e.what());
     return 2;
+  }
+  if (argc > 7) {
+    // a ball resting on a contact point of the fixed shell mesh in argv[3]
+    // (System::importMesh), centred at argv[4..6] with radius argv[7]
+    try {
+      System* s3 = new System(1);
+      dynamic_cast<APGD*>(s3->solver)->setBuild(build);
+      s3->setTimeStep(1e-3);
+      s3->solver->tolerance = 1e-8;
+      s3->solver->maxIterations = 10000;
+      Body* ball = new Body(make_vec3(std::atof(argv[4]), std::atof(argv[5]), std::atof(argv[6])));
+      ball->setGeometry(make_vec3(std::atof(argv[7]), 0, 0));
+      ball->setMass(1.0);
+      s3->add(ball);
+      s3->importMesh(argv[3], 2e8, 8, 0.0);
+      s3->initializeSystem();
+      s3->DoTimeStep();
+      std::vector<double> g3 = s3->gamma(), p3 = s3->p();
+      std::printf("shell_collisions %d shell_gamma %.17g shell_dofs %zu\n", s3->collisionDetector->numCollisions,
+                  g3[0], p3.size());
+      delete s3;
+      delete ball;
+    } catch (const dvide::Error& e) {
+      std::fprintf(stderr, "error %d: %s\n", e.code, e.what());
+      return 3;
+    }
   }
   try {
     System pdip(2);

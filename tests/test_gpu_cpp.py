@@ -2,7 +2,8 @@
 reference's models/test_oneBall.cu, in every APGD build: the step's known
 answers (gamma_n = m g h, f_contact = m g), getCCPViolation of the converged
 contact, System::exportMatrices against the contact's closed-form Jacobian,
-System::importSystem's masses, and the factory's refusal of PDIP."""
+System::importSystem's masses, the factory's refusal of PDIP, and a ball
+resting on a contact point of a fixed shell read by System::importMesh."""
 import os
 import re
 import subprocess
@@ -75,3 +76,27 @@ def test_cpp_one_ball(gpu, build, tmp_path):
 
     # System(2) is PDIP in the reference: not on this path
     assert "factory_pdip -5" in r.stdout
+
+
+def test_cpp_ball_on_an_imported_shell(gpu, tmp_path):
+    """System::importMesh on the C++ surface: a ball of 1 kg resting on top of
+    one contact point of a fixed flat mesh carries gamma_n = m g h, and the
+    DOF vectors hold the ball's 3 DOFs and the mesh's 9 per node."""
+    from dvide_amd import scenes
+    from tests.helpers import shell_point_grid
+
+    mesh = scenes.flat_shell(2, 2, 0.1, 0.1, 0.01, points=8)
+    path = tmp_path / "mesh.txt"
+    scenes.write_mesh(mesh, str(path))
+    pts, _, _ = shell_point_grid(mesh)
+    x, y, z = (float(c) for c in pts[17])
+    r = 0.025
+    args = [str(path), repr(x), repr(y + 0.005 + r), repr(z), repr(r)]
+    for build in (system.BUILD_FP64, system.BUILD_PLAIN):
+        out = subprocess.run([DRIVER, str(build), str(tmp_path)] + args, capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr
+        vals = dict(re.findall(r"(\w+) ([-0-9.e+]+)", out.stdout))
+        assert int(vals["shell_collisions"]) == 1
+        assert int(vals["shell_dofs"]) == 3 + 9 * mesh.n_nodes
+        # within the binary32 rounding of the touching distance (a few 1e-10 m / h)
+        assert float(vals["shell_gamma"]) == pytest.approx(1.0 * 9.81 * 1e-3, rel=1e-3)

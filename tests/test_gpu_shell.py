@@ -157,6 +157,16 @@ def test_free_shell_meshes_and_shell_dof_rows_are_refused(gpu, tmp_path):
         s.importMesh(str(path), 2e8, 8, 0.0)
     assert e.value.code == system.DV_ERR_UNSUPPORTED
     s.close()
+    # malformed files: a truncated one, and a header naming 2^30 nodes
+    scenes.write_mesh(sc.shell, str(path))
+    lines = path.read_text().splitlines()
+    for bad in (lines[:len(lines) // 2], ["1073741824,1,0,0"] + lines[1:]):
+        path.write_text("\n".join(bad) + "\n")
+        s = system.System(system.SOLVER_APGD, system.BUILD_FP64)
+        with pytest.raises(system.DvideError) as e:
+            s.importMesh(str(path), 2e8, 8, 0.0)
+        assert e.value.code == system.DV_ERR_ARG
+        s.close()
     mesh.inv_mass = (np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0))
     s = system.System(system.SOLVER_APGD, system.BUILD_FP64)
     s.add_bodies(sc.pos, sc.vel, sc.mass, sc.geom, sc.fixed, sc.family)
