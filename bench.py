@@ -585,6 +585,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-scene batched solve")
     ap.add_argument("--no-serve", action="store_true", help="skip the served request stream")
     ap.add_argument("--no-shell", action="store_true", help="skip the spheres-on-a-fixed-shell leg")
+    ap.add_argument("--only-shell", action="store_true",
+                    help="run only the spheres-on-a-fixed-shell leg and print its figures")
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the batched leg (with --batch-scenes scenes) and print its figures")
     ap.add_argument("--batch-scenes", type=int, default=512)
@@ -609,6 +611,10 @@ def main():
     if args.only_batch:
         progress(f"{args.batch_scenes}-scene batch")
         print(json.dumps({"batched": batched(n_scenes=args.batch_scenes)}), flush=True)
+        return
+    if args.only_shell:
+        progress("shell-bed leg")
+        print(json.dumps({"shell_bed": shell_bed_leg()}), flush=True)
         return
     progress("settling the compaction pile")
     scene, settle_its = settled_scene(args.settle)
