@@ -208,3 +208,26 @@ def test_batch_with_a_shell_scene_equals_alone(gpu):
             for which in (system.VEC_GAMMA, system.VEC_V, system.VEC_P, system.VEC_F_CONTACT):
                 assert bits_equal_or_nan(x.vector(which), y.vector(which)), which
     batch.close()
+
+
+def test_full_size_shell_bed_plain_bit_exact(gpu):
+    """The bench's shell workload at full size (19,200 spheres, 57,600 contact
+    points): the landed pile (400 mixed steps on the GPU) stepped once by the
+    plain build and by the oracle, contacts, gamma, v and p bit for bit."""
+    import copy
+
+    sc = scenes.shell_bed(3003, nx=20, ny=80, per_side=(24, 100), layers=8, tol=1e-4)
+    g = system.from_scene(sc, system.BUILD_MIXED, tol=1e-4)
+    for _ in range(400):
+        g.step()
+    nb = sc.n
+    landed = copy.copy(sc)
+    landed.pos = g.p[:3 * nb].reshape(-1, 3).copy()
+    landed.vel = g.v[:3 * nb].reshape(-1, 3).copy()
+    g.close()
+    o, g = both(landed, system.BUILD_PLAIN, tol=1e-4)
+    o.step()
+    g.step()
+    assert o.num_contacts > 15000
+    _check_step(o, g, landed, exact=True)
+    g.close()
