@@ -487,6 +487,10 @@ def _one_step_errors(scene, oracle_first, builds, converged=None):
     from dvide_amd import system
 
     free = np.repeat(np.asarray(scene.fixed) == 0, 3)
+    n3 = free.size  # the bodies' DOFs (a shell's follow them)
+    oracle_first = dict(oracle_first, v=oracle_first["v"][:n3], f_contact=oracle_first["f_contact"][:n3])
+    if converged is not None:
+        converged = dict(converged, v=converged["v"][:n3])
     o_asm = oracle.OracleSystem(scene, tol=TOL)
     o_asm.assemble()
     out = {}
@@ -499,12 +503,12 @@ def _one_step_errors(scene, oracle_first, builds, converged=None):
             "iterations": st.iterations,
             "residual": float(f"{st.residual:.3g}"),
             "residual_by_oracle": float(f"{o_asm.residual_of(g.gamma):.3g}"),
-            "rel_v": float(f"{_rel(g.v[free], oracle_first['v'][free]):.3g}"),
-            "rel_f_contact": float(f"{_rel(g.f_contact[free], oracle_first['f_contact'][free]):.3g}"),
+            "rel_v": float(f"{_rel(g.v[:n3][free], oracle_first['v'][free]):.3g}"),
+            "rel_f_contact": float(f"{_rel(g.f_contact[:n3][free], oracle_first['f_contact'][free]):.3g}"),
             "rel_gamma": float(f"{_rel(g.gamma, oracle_first['gamma']):.3g}"),
         }
         if converged is not None:
-            out[name]["rel_v_vs_converged"] = float(f"{_rel(g.v[free], converged['v'][free]):.3g}")
+            out[name]["rel_v_vs_converged"] = float(f"{_rel(g.v[:n3][free], converged['v'][free]):.3g}")
         g.close()
     if converged is not None:
         out["oracle_rel_v_vs_converged"] = float(f"{_rel(oracle_first['v'][free], converged['v'][free]):.3g}")
@@ -554,6 +558,16 @@ def error_check(headline_scene, headline_first):
         fam[f"member_{i}"] = _one_step_errors(sc, _oracle_first_step(sc), builds,
                                               _oracle_first_step(sc, tol=1e-6, max_iterations=8000))
     rep["batch_members"] = fam
+    # spheres on a fixed shell (the shell_bed leg's pile once landed: 400 mixed
+    # steps on the GPU), one step from that state
+    sb = scenes.shell_bed(3003, nx=20, ny=80, per_side=(24, 100), layers=8, tol=TOL)
+    g = system.from_scene(sb, system.BUILD_MIXED, tol=TOL, max_iterations=REFERENCE_CAP)
+    for _ in range(400):
+        g.step()
+    sb.pos, sb.vel = g.p[:3 * sb.n].reshape(-1, 3).copy(), g.v[:3 * sb.n].reshape(-1, 3).copy()
+    g.close()
+    rep["shell_bed_19k"] = _one_step_errors(sb, _oracle_first_step(sb), builds,
+                                            _oracle_first_step(sb, tol=1e-6, max_iterations=20000))
     # gamma on the single layer (unique), at 1e-4 and against the converged 1e-8 solution
     lay = scenes.layer(7)
     rep["layer_256_gamma_unique"] = _one_step_errors(lay, _oracle_first_step(lay), builds)
