@@ -231,3 +231,23 @@ def test_full_size_shell_bed_plain_bit_exact(gpu):
     assert o.num_contacts > 15000
     _check_step(o, g, landed, exact=True)
     g.close()
+
+
+def test_rigid_body_formats_refuse_a_shell_scene(gpu, tmp_path):
+    """export_system and export_collisions write rigid bodies only, and
+    export_matrices needs k, which a shell scene does not give."""
+    from dvide_amd import io
+
+    g = system.from_scene(bed(12), system.BUILD_MIXED)
+    g.step()
+    with pytest.raises(NotImplementedError):
+        io.export_system(g, str(tmp_path / "system.csv"))
+    with pytest.raises(NotImplementedError):
+        io.export_collisions(g, str(tmp_path / "collisions.csv"))
+    with pytest.raises(system.DvideError) as e:
+        io.export_matrices(g, str(tmp_path / "mtx"))
+    assert e.value.code == system.DV_ERR_UNSUPPORTED
+    # the per-collision force output has no per-body layout and still works
+    io.output_contact_force_per_collision(g, str(tmp_path / "forces.csv"))
+    assert (tmp_path / "forces.csv").read_text().count("\n") == 1 + g.numCollisions  # a header line, then one per contact
+    g.close()
