@@ -236,6 +236,30 @@ def shell_bed_leg(settle=400, steps=20):
     return out
 
 
+def fp64_leg(scene, steps, warmup, cap):
+    """The FP64 graph-driven solver alone on the compaction pile: time-steps/s,
+    its P1 launches burst-timed over the timed steps, its P2 launches over two
+    more steps from the same state, and P1's algorithmic bytes against HBM peak
+    (248 B per contact after an accepted iterate, 192 after a backtrack,
+    weighted by the share of fresh passes)."""
+    from dvide_amd import system
+
+    r = run_gpu(scene, system.BUILD_FP64, steps, warmup, profile_kernel=system.PROFILE_P1, max_iterations=cap)
+    p2 = run_gpu(scene, system.BUILD_FP64, 2, 0, profile_kernel=system.PROFILE_P2, max_iterations=cap)
+    nc = int(np.mean(r["contacts"]))
+    p1_us = 1e3 * r["kernel_ms"] / max(r["kernel_count"], 1)
+    p2_us = 1e3 * p2["kernel_ms"] / max(p2["kernel_count"], 1)
+    fresh = float(np.sum(r["iterations"])) / max(float(np.sum(r["passes"])), 1.0)
+    b = fresh * p1_bytes_per_contact(False, True) + (1 - fresh) * p1_bytes_per_contact(False, False)
+    gbs = b * nc / (p1_us * 1e-6) / 1e9 if p1_us > 0 else 0.0
+    return {"time_steps_per_s": round(steps / r["elapsed"], 3), "steps": steps, "contacts_mean": nc,
+            "bodies": int(r["nb"]), "iterations_mean": round(float(np.mean(r["iterations"])), 1),
+            "passes_mean": round(float(np.mean(r["passes"])), 1), "fresh_share": round(fresh, 3),
+            "P1_us": round(p1_us, 3), "P2_us": round(p2_us, 3),
+            "p1_bytes_per_contact": round(b, 1), "p1_achieved_GBs": round(gbs, 1),
+            "p1_frac_of_8TBs": round(gbs / 8000.0, 4), "schedule": r["schedule"]}
+
+
 def _gen_many(specs):
     """Scenes generated on the host cores in parallel (the generator releases
     the interpreter lock)."""
@@ -601,6 +625,8 @@ def main():
     ap.add_argument("--no-shell", action="store_true", help="skip the spheres-on-a-fixed-shell leg")
     ap.add_argument("--only-shell", action="store_true",
                     help="run only the spheres-on-a-fixed-shell leg and print its figures")
+    ap.add_argument("--only-fp64", action="store_true",
+                    help="run only the FP64 graph-driven solver on the settled pile and print its figures")
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the batched leg (with --batch-scenes scenes) and print its figures")
     ap.add_argument("--batch-scenes", type=int, default=512)
@@ -632,6 +658,10 @@ def main():
         return
     progress("settling the compaction pile")
     scene, settle_its = settled_scene(args.settle)
+    if args.only_fp64:
+        progress("FP64 graph-driven solver")
+        print(json.dumps({"fp64": fp64_leg(scene, args.steps, args.warmup, args.max_iterations)}), flush=True)
+        return
     progress("mixed solver")
     cap = args.max_iterations
     mixed = run_gpu(scene, system.BUILD_MIXED, args.steps, args.warmup, profile_kernel=system.PROFILE_P1,
