@@ -362,6 +362,10 @@ int dv_add_shell_mesh(dv_scene* h, int n_nodes, const double* nodes, int n_eleme
     if (n_nodes <= 0 || n_elements <= 0 || !nodes || !connectivity || !geometry)
       return dv::fail(s, DV_ERR_ARG, "bad shell mesh arrays");
     if (points_per_side < 2) return dv::fail(s, DV_ERR_ARG, "points per element side must be at least 2");
+    // the points join the bodies, and a node's incidence list packs (point, corner) into an int
+    const long side = (long)points_per_side - 2;
+    if (side > 8192 || side * side * (long)n_elements > dv::kMaxShellPoints)
+      return dv::fail(s, DV_ERR_ARG, "more than 2^26 shell contact points");
     for (int e = 0; e < n_elements; ++e) {
       for (int q = 0; q < 4; ++q)
         if (connectivity[4 * e + q] < 0 || connectivity[4 * e + q] >= n_nodes)

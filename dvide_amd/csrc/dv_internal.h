@@ -70,6 +70,9 @@ constexpr int kHeavyDegree = 128;
 
 constexpr int kSlots = 8;  // reduction slots of the solver kernels
 
+// Most contact points a shell mesh may add (elements x (points per side - 2)^2).
+constexpr long kMaxShellPoints = 1L << 26;
+
 struct ApgdState;
 struct SolveDims;
 struct Scene;

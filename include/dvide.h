@@ -165,7 +165,8 @@ int dv_add_bilateral_dof(dv_scene* s, int dofA, int dofB, double velocity, doubl
  * are supported: a mesh with a nonzero M^-1 entry (a shell free to move,
  * which needs the shell's elastic forces) fails with DV_ERR_UNSUPPORTED, and
  * stiffness and pressure, which act on free DOFs only, are not used.  One
- * mesh per scene, after the bodies and before dv_initialize. */
+ * mesh per scene, after the bodies and before dv_initialize; at most 2^24
+ * nodes or elements and 2^26 contact points. */
 int dv_import_mesh(dv_scene* s, const char* path, double stiffness, int points, double pressure);
 
 /* The same fixed shell mesh from arrays (no counterpart in the reference;
@@ -173,7 +174,8 @@ int dv_import_mesh(dv_scene* s, const char* path, double stiffness, int points, 
  * y-gradient per node), connectivity[4*n_elements] (0-based),
  * geometry[4*n_elements] (a, b, thickness, ignored), `points` contact points
  * per element side (as importMesh's numContactPointsPerElement), and fext[9*
- * n_nodes] (may be NULL: zeros), which enters the potential energy only. */
+ * n_nodes] (may be NULL: zeros), which enters the potential energy only.
+ * At most 2^26 contact points per mesh (DV_ERR_ARG beyond). */
 int dv_add_shell_mesh(dv_scene* s, int n_nodes, const double* nodes, int n_elements, const int* connectivity,
                       const double* geometry, int points, const double* fext);
 

@@ -9,6 +9,8 @@ is gathered point by point, a different order of the same sums: 1e-12.  D's
 3x36 shell blocks are exported exactly as the oracle builds them.  The
 graph-driven builds are held to the oracle's solution as on other scenes.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -168,6 +170,13 @@ def test_free_shell_meshes_and_shell_dof_rows_are_refused(gpu, tmp_path):
         assert e.value.code == system.DV_ERR_ARG
         s.close()
     mesh.inv_mass = (np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    # more than 2^26 contact points (8,194 per side: 8,192^2 interior points per element)
+    s = system.System(system.SOLVER_APGD, system.BUILD_FP64)
+    wide = dataclasses.replace(mesh, points=8194)
+    with pytest.raises(system.DvideError) as e:
+        s.add_shell_mesh(wide)
+    assert e.value.code == system.DV_ERR_ARG
+    s.close()
     s = system.System(system.SOLVER_APGD, system.BUILD_FP64)
     s.add_bodies(sc.pos, sc.vel, sc.mass, sc.geom, sc.fixed, sc.family)
     s.add_shell_mesh(mesh)
