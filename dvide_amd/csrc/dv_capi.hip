@@ -97,6 +97,9 @@ void initialize(Scene& s) {
   for (auto& e : s.ev)
     if (!e) DV_HIP(hipEventCreate(&e));
   if (!s.pinned) DV_HIP(hipHostMalloc(&s.pinned, 16 * sizeof(int), hipHostMallocDefault));
+  const long side = s.nNodes ? s.shellPoints - 2 : 0, nPts = (long)(s.hShellConn.size() / 4) * side * side;
+  if ((long)s.hMass.size() + nPts > kMaxBodies)
+    throw DvError{DV_ERR_UNSUPPORTED, "more than 2^28 bodies and shell contact points in one scene"};
   s.nUser = (int)s.hMass.size();
   if (s.nNodes) shellAppendPoints(s);  // the shells' contact points follow the caller's bodies
   const int nb = (int)s.hMass.size();

@@ -459,6 +459,8 @@ void collideEmit(Scene& s) {
   DV_HIP(hipStreamSynchronize(st));  // the counts of collideCount
   const int nc = s.pinned[0];
   const int ninc = s.pinned[1];
+  if (nc < 0 || ninc < 0 || nc > kMaxContacts)
+    throw DvError{DV_ERR_UNSUPPORTED, "more than 2^28 contacts in one scene"};
   if (ninc != 2 * nc) throw DvError{DV_ERR_STATE, "collision detection: asymmetric pair counts"};
   s.nc = nc;
   if (nc > s.ncCap || s.ncCap == 0) {

@@ -72,6 +72,10 @@ constexpr int kSlots = 8;  // reduction slots of the solver kernels
 
 // Most contact points a shell mesh may add (elements x (points per side - 2)^2).
 constexpr long kMaxShellPoints = 1L << 26;
+// Most bodies (shell contact points included) and contacts of one scene: row
+// and DOF offsets (3 per body or contact, 2 incidences per contact) are ints.
+constexpr long kMaxBodies = 1L << 28;
+constexpr long kMaxContacts = 1L << 28;
 
 struct ApgdState;
 struct SolveDims;

@@ -186,11 +186,15 @@ int dv_set_bins_per_axis(dv_scene* s, int bx, int by, int bz);
 int dv_set_param(dv_scene* s, int param, double value);
 int dv_get_param(const dv_scene* s, int param, double* value);
 
-/* System::initializeSystem, System.cu:342-508: packs and uploads the bodies. */
+/* System::initializeSystem, System.cu:342-508: packs and uploads the bodies.
+ * A scene holds at most 2^28 bodies, shell contact points included
+ * (DV_ERR_UNSUPPORTED beyond). */
 int dv_initialize(dv_scene* s);
 
 /* System::DoTimeStep, System.cu:544-602: collision detection, D and r
- * assembly, APGD, and the update v = M^-1 (k + D^T gamma), p += h v. */
+ * assembly, APGD, and the update v = M^-1 (k + D^T gamma), p += h v.  A step
+ * whose collision detection finds more than 2^28 contacts fails with
+ * DV_ERR_UNSUPPORTED. */
 int dv_step(dv_scene* s);
 
 /* --- data access (System's public device vectors) ------------------------- */
