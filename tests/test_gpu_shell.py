@@ -260,3 +260,30 @@ def test_rigid_body_formats_refuse_a_shell_scene(gpu, tmp_path):
     io.output_contact_force_per_collision(g, str(tmp_path / "forces.csv"))
     assert (tmp_path / "forces.csv").read_text().count("\n") == 1 + g.numCollisions  # a header line, then one per contact
     g.close()
+
+
+@pytest.mark.parametrize("points", [2, 3])
+def test_meshes_with_no_point_or_one_per_element(gpu, points):
+    """importMesh's smallest sampling: 2 points per side puts no contact point
+    on an element (the spheres fall freely), 3 puts one at its centre."""
+    sc = scenes.shell_bed(11, nx=4, ny=4, per_side=(4, 4), layers=2, points=points)
+    o, g = both(sc, system.BUILD_PLAIN, tol=1e-8)
+    nb = sc.n
+    for _ in range(2):
+        o.step()
+        g.step()
+        a0, b0, _, _ = o.contacts()
+        a1, b1, _, _ = g.contacts()
+        assert np.array_equal(pair_keys(a0, b0), pair_keys(a1, b1))
+        assert int((b1 >= nb).sum()) == (0 if points == 2 else 4)
+        assert g.stats().iterations == o.iterations
+        for which in (oracle.GAMMA, oracle.V, oracle.P):
+            assert bits_equal_or_nan(o.vec(which), g.vector(which)), which
+        fo, fg = o.vec(oracle.F_CONTACT), g.f_contact
+        assert fg.shape == (3 * nb + 9 * sc.shell.n_nodes,)
+        assert bits_equal_or_nan(fo[:3 * nb], fg[:3 * nb])
+        scale = max(np.abs(fo[3 * nb:]).max(), 1e-300)
+        assert np.abs(fo[3 * nb:] - fg[3 * nb:]).max() <= 1e-12 * scale
+        if points == 2:
+            assert not fg[3 * nb:].any()
+    g.close()
