@@ -814,7 +814,10 @@ struct System {
   void projectedIterations() {
     const double omega = 0.3, lambda = 2.0 / 3.0;  // PGS.cu:13-14, PGJ.cu:14-15
     iterations = -1;
-    if (!bilA.empty() || nNodes) return;  // contact rows between bodies only
+    if (!bilA.empty() || nNodes) {  // contact rows between bodies only
+      gamma.assign(rows(), 0.0);     // refused: the step goes on with no impulses
+      return;
+    }
     gamma.assign(3 * (size_t)nc, 0.0);
     for (int c = 0; c < nc; ++c) gamma[3 * c] = 1.0;
     std::vector<double> B, w, gh;

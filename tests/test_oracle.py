@@ -372,5 +372,8 @@ def test_projected_solvers_refuse_bilateral_rows(one_thread):
     p = oracle.OracleSystem(sc, tol=1e-6, max_iterations=10, solver=oracle.PGS)
     p.step()
     assert p.iterations == -1
+    # the refused step goes on with no impulses (a gamma of one zero per row)
+    g = p.vec(oracle.GAMMA)
+    assert not g.any() and np.isfinite(p.vec(oracle.V)).all()
     with pytest.raises(ValueError):
         oracle.OracleSystem(sc, solver=2)
